@@ -1,0 +1,354 @@
+/*
+ * ws_oracle.c -- CPU FP64 oracle for the inviscid-RHS path (plain C, OpenMP).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py load this library, and only as the checker.
+ * The shipped HIP library never links or calls it.
+ *
+ * Restates the reference's periodic (D0_bound = 0) semantics of the
+ * decomposed WENO7-SYMBO path:
+ *   split      src/solver/OCFD_NS_Jacobian3d.cpp:69-127
+ *   WENO       src/kernels/OCFD_Schemes.cpp:306-368, 491-553
+ *   faces/du   src/kernels/OCFD_Schemes.cpp:221-296, 461-484, 639-662
+ *   prims      src/kernels/commen_kernel.cpp:32-66, 78-110, 145-154
+ *   halo       src/mpi/OCFD_mpi_dev.cpp:187-290 with src/mpi/OCFD_mpi.c:171-197
+ *   RK3        src/solver/OCFD_time.cpp:15-65, loop src/solver/OCFD_NS_Solver.cpp:64-90
+ *   OCFD_bc    src/boundary/OCFD_boundary.cpp:38-89
+ * Build with -ffp-contract=off so that every operation rounds as written.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "ws_oracle.h"
+#include "ws_oracle_cell.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define LAP 4
+
+static inline size_t gx(const orc_params *p) { return (size_t)p->nx + 2 * LAP; }
+static inline size_t gy(const orc_params *p) { return (size_t)p->ny + 2 * LAP; }
+static inline size_t gz(const orc_params *p) { return (size_t)p->nz + 2 * LAP; }
+
+/* ghosted index, (i,j,k) in interior coordinates, each may lie in [-LAP, n+LAP) */
+static inline size_t G(const orc_params *p, long i, long j, long k)
+{
+    return (size_t)(i + LAP) + gx(p) * ((size_t)(j + LAP) + gy(p) * (size_t)(k + LAP));
+}
+
+/* interior SoA index */
+static inline size_t S(const orc_params *p, long i, long j, long k, int m)
+{
+    return (size_t)i + (size_t)p->nx * ((size_t)j + (size_t)p->ny * ((size_t)k + (size_t)p->nz * (size_t)m));
+}
+
+static void set_threads(const orc_params *p)
+{
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#else
+    (void)p;
+#endif
+}
+
+double orc_weno7_symbo_p(const double *s8) { return orc_cell_weno7_symbo_p(s8); }
+double orc_weno7_symbo_m(const double *s8) { return orc_cell_weno7_symbo_m(s8); }
+
+void orc_weno7_symbo_p_parts(const double *s8, double Sv[5], double wts[5], double q[5])
+{
+    double parts[15];
+    int k;
+    (void)orc_cell_weno7_symbo_p_ex(s8, parts);
+    for (k = 0; k < 5; k++) {
+        Sv[k] = parts[k];
+        wts[k] = parts[5 + k];
+        q[k] = parts[10 + k];
+    }
+}
+
+void orc_split_sw(double d, double u, double v, double w, double cc,
+                  double A1, double A2, double A3,
+                  double gamma, double epsl_sw, double fp[5], double fm[5])
+{
+    const double C1 = 2.0 * (gamma - 1.0);
+    const double C3 = (3.0 - gamma) / (2.0 * (gamma - 1.0));
+    orc_cell_split_sw(d, u, v, w, cc, A1, A2, A3, gamma, epsl_sw, C1, C3, fp, fm);
+}
+
+/* ---------------------------------------------------------------- prims */
+
+static double cv_of(const orc_params *p)
+{
+    /* src/runtime/OCFD_init.cpp:380 */
+    return 1.0 / (p->gamma * (p->gamma - 1.0) * p->ama * p->ama);
+}
+
+void orc_cons_to_pri(const orc_params *p, const double *f,
+                     double *d, double *u, double *v, double *w, double *T, double *P)
+{
+    const double Cv = cv_of(p);
+    long k;
+    set_threads(p);
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < p->nz; k++) {
+        long i, j;
+        for (j = 0; j < p->ny; j++)
+            for (i = 0; i < p->nx; i++) {
+                double out[6];
+                size_t g = G(p, i, j, k);
+                orc_cell_cons_to_pri(f[S(p, i, j, k, 0)], f[S(p, i, j, k, 1)], f[S(p, i, j, k, 2)],
+                                     f[S(p, i, j, k, 3)], f[S(p, i, j, k, 4)], Cv, p->gamma, out);
+                d[g] = out[0]; u[g] = out[1]; v[g] = out[2]; w[g] = out[3]; T[g] = out[4];
+                if (P) P[g] = out[5];
+            }
+    }
+}
+
+void orc_pri_to_cons_box(const orc_params *p, double *f,
+                         const double *d, const double *u, const double *v, const double *w, const double *T,
+                         const int start[3], const int end[3])
+{
+    const double Cv = cv_of(p);
+    long i, j, k;
+    for (k = start[2]; k < end[2]; k++)
+        for (j = start[1]; j < end[1]; j++)
+            for (i = start[0]; i < end[0]; i++) {
+                double c[5];
+                size_t g = G(p, i, j, k);
+                int m;
+                orc_cell_pri_to_cons(d[g], u[g], v[g], w[g], T[g], Cv, c);
+                for (m = 0; m < 5; m++) f[S(p, i, j, k, m)] = c[m];
+            }
+}
+
+/* Periodic LAP-deep face halo: x, then y, then z, each over the interior of
+ * the other two dimensions (edges and corners are left untouched, as the
+ * packed exchange does: src/mpi/OCFD_mpi_dev.cpp:187-290). */
+void orc_ghost_fill_periodic(const orc_params *p, double *fld)
+{
+    const long nx = p->nx, ny = p->ny, nz = p->nz;
+    long i, j, k;
+    for (k = 0; k < nz; k++)
+        for (j = 0; j < ny; j++)
+            for (i = 0; i < LAP; i++) {
+                fld[G(p, nx + i, j, k)] = fld[G(p, i, j, k)];          /* right ghost <- left interior */
+                fld[G(p, i - LAP, j, k)] = fld[G(p, nx - LAP + i, j, k)]; /* left ghost <- right interior */
+            }
+    for (k = 0; k < nz; k++)
+        for (j = 0; j < LAP; j++)
+            for (i = 0; i < nx; i++) {
+                fld[G(p, i, ny + j, k)] = fld[G(p, i, j, k)];
+                fld[G(p, i, j - LAP, k)] = fld[G(p, i, ny - LAP + j, k)];
+            }
+    for (k = 0; k < LAP; k++)
+        for (j = 0; j < ny; j++)
+            for (i = 0; i < nx; i++) {
+                fld[G(p, i, j, nz + k)] = fld[G(p, i, j, k)];
+                fld[G(p, i, j, k - LAP)] = fld[G(p, i, j, nz - LAP + k)];
+            }
+}
+
+/* cc over the whole ghosted box (du_invis_Jacobian3d_init launches the
+ * sound-speed kernel on [start-LAP, end+LAP), OCFD_NS_Jacobian3d.cpp:24-33) */
+void orc_sound_speed(const orc_params *p, const double *T, double *cc)
+{
+    const size_t n = gx(p) * gy(p) * gz(p);
+    size_t q;
+    for (q = 0; q < n; q++) cc[q] = orc_cell_sound_speed(T[q], p->ama);
+}
+
+/* ---------------------------------------------------------------- RHS */
+
+/* One direction of the RHS on one line: split at the n+2*LAP points, then the
+ * P faces for all five variables (OCFD_weno7_SYMBO_P_kernel + put_du_p_kernel),
+ * then the M faces (OCFD_weno7_SYMBO_M_kernel + put_du_m_kernel).
+ * Face f (f = 0..n) sits between cells f-1 and f and reconstructs from the
+ * split values at f-4 .. f+3.  prim[]/metric pointers are pre-offset per
+ * point by the caller through the `at` index array. */
+static void rhs_line(const orc_params *p, long n, const size_t *at, const size_t *du_at, double h,
+                     const double *d, const double *u, const double *v, const double *w, const double *cc,
+                     const double *A1, const double *A2, const double *A3, const double *ajac,
+                     double *du, double *buf)
+{
+    const double C1 = 2.0 * (p->gamma - 1.0);
+    const double C3 = (3.0 - p->gamma) / (2.0 * (p->gamma - 1.0));
+    const long np = n + 2 * LAP;
+    double *fpv = buf;              /* [5][np] */
+    double *fmv = buf + 5 * np;     /* [5][np] */
+    double *hP = buf + 10 * np;     /* [n+1] */
+    double *hM = hP + (n + 1);      /* [n+1] */
+    long q, f, c;
+    int m;
+
+    for (q = 0; q < np; q++) {
+        double fp5[5], fm5[5];
+        size_t g = at[q];
+        orc_cell_split_sw(d[g], u[g], v[g], w[g], cc[g], A1[g], A2[g], A3[g],
+                          p->gamma, p->epsl_sw, C1, C3, fp5, fm5);
+        for (m = 0; m < 5; m++) { fpv[m * np + q] = fp5[m]; fmv[m * np + q] = fm5[m]; }
+    }
+    for (m = 0; m < 5; m++) {
+        for (f = 0; f <= n; f++) hP[f] = orc_cell_weno7_symbo_p(&fpv[m * np + f]);
+        for (c = 0; c < n; c++) {
+            size_t g = at[c + LAP];
+            du[du_at[c] + m] = orc_cell_flux_diff(du[du_at[c] + m], ajac[g], hP[c + 1], hP[c], h);
+        }
+    }
+    for (m = 0; m < 5; m++) {
+        for (f = 0; f <= n; f++) hM[f] = orc_cell_weno7_symbo_m(&fmv[m * np + f]);
+        for (c = 0; c < n; c++) {
+            size_t g = at[c + LAP];
+            du[du_at[c] + m] = orc_cell_flux_diff(du[du_at[c] + m], ajac[g], hM[c + 1], hM[c], h);
+        }
+    }
+}
+
+/* The du of each line is gathered into a small [n][5] buffer, updated by
+ * rhs_line and scattered back, so that rhs_line is direction-agnostic. */
+static void rhs_direction(const orc_params *p, int dir,
+                          const double *d, const double *u, const double *v, const double *w, const double *cc,
+                          const double *const metrics[9], const double *ajac, double *du)
+{
+    const long nx = p->nx, ny = p->ny, nz = p->nz;
+    const long n = dir == 0 ? nx : dir == 1 ? ny : nz;
+    const long n1 = dir == 0 ? ny : nx;            /* the two other dims, outer loop last */
+    const long n2 = dir == 2 ? ny : nz;
+    const double h = dir == 0 ? p->hx : dir == 1 ? p->hy : p->hz;
+    const size_t N = (size_t)nx * ny * nz;
+    const double *A1 = metrics[3 * dir + 0];
+    const double *A2 = metrics[3 * dir + 1];
+    const double *A3 = metrics[3 * dir + 2];
+    long b;
+    set_threads(p);
+#pragma omp parallel
+    {
+        const long np = n + 2 * LAP;
+        size_t *at = (size_t *)malloc(sizeof(size_t) * np);
+        size_t *du_at = (size_t *)malloc(sizeof(size_t) * n);
+        double *buf = (double *)malloc(sizeof(double) * (10 * np + 2 * (n + 1)));
+        double *dl = (double *)malloc(sizeof(double) * 5 * n);   /* du of this line, [n][5] */
+#pragma omp for schedule(static)
+        for (b = 0; b < n1 * n2; b++) {
+            long a1 = b % n1, a2 = b / n1, q, c;
+            int m;
+            for (q = 0; q < np; q++) {
+                long t = q - LAP;
+                at[q] = dir == 0 ? G(p, t, a1, a2) : dir == 1 ? G(p, a1, t, a2) : G(p, a1, a2, t);
+            }
+            for (c = 0; c < n; c++) {
+                size_t base = dir == 0 ? S(p, c, a1, a2, 0) : dir == 1 ? S(p, a1, c, a2, 0) : S(p, a1, a2, c, 0);
+                du_at[c] = (size_t)(5 * c);
+                for (m = 0; m < 5; m++) dl[5 * c + m] = du[base + (size_t)m * N];
+            }
+            rhs_line(p, n, at, du_at, h, d, u, v, w, cc, A1, A2, A3, ajac, dl, buf);
+            for (c = 0; c < n; c++) {
+                size_t base = dir == 0 ? S(p, c, a1, a2, 0) : dir == 1 ? S(p, a1, c, a2, 0) : S(p, a1, a2, c, 0);
+                for (m = 0; m < 5; m++) du[base + (size_t)m * N] = dl[5 * c + m];
+            }
+        }
+        free(at); free(du_at); free(buf); free(dl);
+    }
+}
+
+void orc_inviscid_rhs(const orc_params *p,
+                      const double *d, const double *u, const double *v, const double *w, const double *cc,
+                      const double *const metrics[9], const double *ajac, double *du)
+{
+    int dir;
+    for (dir = 0; dir < 3; dir++)
+        rhs_direction(p, dir, d, u, v, w, cc, metrics, ajac, du);
+}
+
+void orc_du_comput(const orc_params *p, double *d, double *u, double *v, double *w, double *T, double *cc,
+                   const double *const metrics[9], const double *ajac, double *du)
+{
+    memset(du, 0, sizeof(double) * 5 * (size_t)p->nx * p->ny * p->nz);
+    orc_ghost_fill_periodic(p, d);
+    orc_ghost_fill_periodic(p, u);
+    orc_ghost_fill_periodic(p, v);
+    orc_ghost_fill_periodic(p, w);
+    orc_ghost_fill_periodic(p, T);
+    orc_sound_speed(p, T, cc);
+    orc_inviscid_rhs(p, d, u, v, w, cc, metrics, ajac, du);
+}
+
+/* ---------------------------------------------------------------- time */
+
+void orc_time_advance(const orc_params *p, int krk, double *f, const double *fn, const double *du, double dt)
+{
+    const size_t n = 5 * (size_t)p->nx * p->ny * p->nz;
+    size_t q;
+    if (krk == 1) {
+        for (q = 0; q < n; q++) f[q] = fn[q] + dt * du[q];
+    } else if (krk == 2) {
+        const double tmp1 = 3.0 / 4.0, tmp2 = 1.0 / 4.0;
+        for (q = 0; q < n; q++) f[q] = tmp1 * fn[q] + tmp2 * (f[q] + dt * du[q]);
+    } else {
+        const double tmp1 = 1.0 / 3.0, tmp2 = 2.0 / 3.0;
+        for (q = 0; q < n; q++) f[q] = tmp1 * fn[q] + tmp2 * (f[q] + dt * du[q]);
+    }
+}
+
+void orc_bc_planes(const orc_params *p, double *f,
+                   const double *d, const double *u, const double *v, const double *w, const double *T)
+{
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    int s[3], e[3];
+    /* x = 0, x = nx-1, y = 0, y = ny-1, z = 0, z = nz-1, in that order */
+    s[0] = 0;      s[1] = 0;      s[2] = 0;      e[0] = 1;  e[1] = ny; e[2] = nz; orc_pri_to_cons_box(p, f, d, u, v, w, T, s, e);
+    s[0] = nx - 1;                               e[0] = nx;                       orc_pri_to_cons_box(p, f, d, u, v, w, T, s, e);
+    s[0] = 0;      s[1] = 0;                     e[0] = nx; e[1] = 1;             orc_pri_to_cons_box(p, f, d, u, v, w, T, s, e);
+    s[1] = ny - 1;                               e[1] = ny;                       orc_pri_to_cons_box(p, f, d, u, v, w, T, s, e);
+    s[1] = 0;      s[2] = 0;                     e[1] = ny; e[2] = 1;             orc_pri_to_cons_box(p, f, d, u, v, w, T, s, e);
+    s[2] = nz - 1;                               e[2] = nz;                       orc_pri_to_cons_box(p, f, d, u, v, w, T, s, e);
+}
+
+double orc_cfl_dt(const orc_params *p, double cfl,
+                  const double *u, const double *v, const double *w, const double *T)
+{
+    double smax = 0.0, hmin;
+    long i, j, k;
+    for (k = 0; k < p->nz; k++)
+        for (j = 0; j < p->ny; j++)
+            for (i = 0; i < p->nx; i++) {
+                size_t g = G(p, i, j, k);
+                double c = orc_cell_sound_speed(T[g], p->ama);
+                double a = fabs(u[g]) + c, b = fabs(v[g]) + c, e = fabs(w[g]) + c;
+                smax = fmax(smax, fmax(a, fmax(b, e)));
+            }
+    hmin = fmin(p->hx, fmin(p->hy, p->hz));
+    return cfl * hmin / smax;
+}
+
+void orc_rk3_run(const orc_params *p, int nsteps, double dt,
+                 double *f, double *d, double *u, double *v, double *w, double *T)
+{
+    const size_t N = (size_t)p->nx * p->ny * p->nz;
+    const size_t NG = gx(p) * gy(p) * gz(p);
+    double *fn = (double *)malloc(sizeof(double) * 5 * N);
+    double *du = (double *)malloc(sizeof(double) * 5 * N);
+    double *cc = (double *)calloc(NG, sizeof(double));
+    double *one = (double *)malloc(sizeof(double) * NG);
+    double *zero = (double *)calloc(NG, sizeof(double));
+    const double *metrics[9];
+    size_t q;
+    int step, krk;
+    for (q = 0; q < NG; q++) one[q] = 1.0;
+    metrics[0] = one;  metrics[1] = zero; metrics[2] = zero;
+    metrics[3] = zero; metrics[4] = one;  metrics[5] = zero;
+    metrics[6] = zero; metrics[7] = zero; metrics[8] = one;
+
+    for (step = 0; step < nsteps; step++) {
+        double dts = dt > 0.0 ? dt : orc_cfl_dt(p, -dt, u, v, w, T);
+        /* pf <-> pfn swap: fn holds the state at t^n (OCFD_NS_Solver.cpp:66-74) */
+        memcpy(fn, f, sizeof(double) * 5 * N);
+        for (krk = 1; krk <= 3; krk++) {
+            orc_du_comput(p, d, u, v, w, T, cc, metrics, one, du);
+            orc_time_advance(p, krk, f, fn, du, dts);
+            orc_cons_to_pri(p, f, d, u, v, w, T, NULL);
+            if (p->bc_planes) orc_bc_planes(p, f, d, u, v, w, T);
+        }
+    }
+    free(fn); free(du); free(cc); free(one); free(zero);
+}

@@ -1,0 +1,87 @@
+/*
+ * ws_oracle.h -- CPU FP64 oracle for the inviscid-RHS path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py load this code, and only as a checker; the
+ * shipped library never links it.
+ *
+ * Layouts (all dense, double precision):
+ *   ghosted field   f[(i+4) + NXG*((j+4) + NYG*(k+4))], NXG = nx+8, NYG = ny+8,
+ *                   i in [-4, nx+4): the reference's cudaField with LAP = 4
+ *                   (include/windstencil/cuda_utility.h:21, get_Field_LAP);
+ *   interior SoA    f[i + nx*(j + ny*(k + nz*m))], m = 0..4: the reference's
+ *                   cudaSoA (cuda_utility.h:30, get_SoA).
+ */
+#ifndef WS_ORACLE_H
+#define WS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+    int nx, ny, nz;          /* interior extents */
+    double gamma;            /* ratio of specific heats (reference Gamma) */
+    double ama;              /* reference Mach number (reference Ama) */
+    double epsl_sw;          /* Steger-Warming smoothing epsilon (EPSL_SW) */
+    double hx, hy, hz;       /* grid spacings, 1/(N_global-1) in the reference */
+    int bc_planes;           /* 1: re-derive conserved vars on the 6 boundary planes after each stage (OCFD_bc) */
+    int nthreads;            /* OpenMP threads, 0 = runtime default */
+} orc_params;
+
+/* scalar kernels (for closed-form tests) */
+double orc_weno7_symbo_p(const double *s8);
+double orc_weno7_symbo_m(const double *s8);
+void   orc_split_sw(double d, double u, double v, double w, double cc,
+                    double A1, double A2, double A3,
+                    double gamma, double epsl_sw, double fp[5], double fm[5]);
+
+/* field kernels */
+void orc_cons_to_pri(const orc_params *p, const double *f,
+                     double *d, double *u, double *v, double *w, double *T, double *P);
+void orc_pri_to_cons_box(const orc_params *p, double *f,
+                         const double *d, const double *u, const double *v, const double *w, const double *T,
+                         const int start[3], const int end[3]);
+void orc_ghost_fill_periodic(const orc_params *p, double *fld);
+void orc_sound_speed(const orc_params *p, const double *T, double *cc);
+
+/* Inviscid RHS from ghosted primitives.  metrics[0..8] = Akx,Aky,Akz,Aix,Aiy,Aiz,Asx,Asy,Asz
+ * (ghosted fields), ajac a ghosted field.  du (interior SoA) is ACCUMULATED into, as the
+ * reference's decomposed kernels do; zero it first for a plain RHS. */
+void orc_inviscid_rhs(const orc_params *p,
+                      const double *d, const double *u, const double *v, const double *w, const double *cc,
+                      const double *const metrics[9], const double *ajac, double *du);
+
+/* Full RHS of one RK stage from the current primitives (du_comput):
+ * zero du, periodic ghost fill of d,u,v,w,T, sound speed, inviscid RHS. */
+void orc_du_comput(const orc_params *p, double *d, double *u, double *v, double *w, double *T, double *cc,
+                   const double *const metrics[9], const double *ajac, double *du);
+
+/* SSP-RK3 stage update, src/solver/OCFD_time.cpp:15-65 (f_lap not produced). */
+void orc_time_advance(const orc_params *p, int krk, double *f, const double *fn, const double *du, double dt);
+
+/* OCFD_bc: conserved vars on the 6 boundary planes from primitives. */
+void orc_bc_planes(const orc_params *p, double *f,
+                   const double *d, const double *u, const double *v, const double *w, const double *T);
+
+/* CFL time step: dt = cfl*min(hx,hy,hz)/max_cells max_dir(|u_dir| + c), c = sqrt(T)/Ma. */
+double orc_cfl_dt(const orc_params *p, double cfl,
+                  const double *u, const double *v, const double *w, const double *T);
+
+/* N SSP-RK3 steps with Cartesian constant metrics (Akx=Aiy=Asz=Ajac=1, others 0).
+ * State in/out: f (interior SoA) and the ghosted primitives d,u,v,w,T (the primitives must
+ * correspond to f on entry, as after init or after a previous call).  dt > 0: fixed step;
+ * dt <= 0: CFL step with cfl = -dt, recomputed at the start of every step. */
+void orc_rk3_run(const orc_params *p, int nsteps, double dt,
+                 double *f, double *d, double *u, double *v, double *w, double *T);
+
+/* Internals of one P reconstruction (for first-principles tests): smoothness
+ * indicators after the downwind max, the unnormalised product-form weights
+ * d_k*prod_{j!=k}(eps+S_j) and the five candidate values q_k (before the
+ * division by 12). */
+void orc_weno7_symbo_p_parts(const double *s8, double S[5], double wts[5], double q[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
