@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the inviscid RHS on one MI355X.
+
+A step is one full RHS evaluation L(U) of the 600x256x128 periodic box
+(the Sod-plus-noise workload): on-device ghost fill of rho,u,v,w,T, sound
+speed over the ghosted box, and the three directional WENO7-SYMBO /
+Steger-Warming kernels writing du.  Inputs are resident in HBM before the
+timed region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "inviscid RHS Mcell-updates/s, FP64 vs mixed at 600x256x128; % attainable roof"
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_HBM_GBS = 8000.0
+ALGO_BYTES_PER_CELL = 80     # 5 FP64 state in + 5 FP64 RHS out
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grid", default="600x256x128")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline leg")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (for HBM counter calibration)")
+    return ap.parse_args()
+
+
+# Algorithmic FLOPs of one RHS per cell in the reference's formulation (add, sub,
+# mul, div, sqrt, max = 1; no FMA): 3 directions x (split 137 + 5 x (WENO P 230 +
+# WENO M 230) + 10 flux differences x 4) = 7431.  Counted by the CPU restatement's
+# op-counting build; tests/test_bench_contract.py keeps the two in step.
+ALGO_FLOPS_PER_CELL = 7431
+ALGO_FLOPS_PER_DIRECTION = 2477
+
+
+def algo_flops():
+    return ALGO_FLOPS_PER_CELL, ALGO_FLOPS_PER_DIRECTION
+
+
+def calibrate_copy():
+    import torch
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": n * 8, "bytes_written": n * 8}))
+
+
+def cpu_baseline(prims, nx, ny, nz, budget_s):
+    """The oracle (C restatement, OpenMP) on the host cores, on a bounded sample:
+    whole RHS evaluations of the same state, repeated until the budget is used."""
+    from oracle import oracle as O
+    from tests.helpers import ghosted
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    p = O.params(nx, ny, nz, nthreads=threads)
+    g = [ghosted(a) for a in prims]
+    for a in g:
+        O.ghost_fill_periodic(p, a)
+    cc = O.sound_speed(p, g[4])
+    met = O.metrics_cartesian(p)
+    aj = np.ones_like(cc)
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while True:
+        du = np.zeros((5, nz, ny, nx))
+        t0 = time.perf_counter()
+        O.inviscid_rhs(p, g[0], g[1], g[2], g[3], cc, met, aj, du)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end or len(times) >= 5:
+            break
+    med = statistics.median(times)
+    return {"value": round(nx * ny * nz / med / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(times)} full RHS evaluations of the {nx}x{ny}x{nz} box (median), "
+                      f"C restatement -O3 -fopenmp, {threads} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 "
+                                   "(the multi-GPU decomposition is a design only, see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    import windstencil_amd as W
+    from windstencil_amd import initial as I
+
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    N = nx * ny * nz
+    L = W.lib()
+    prec = W.WS_PREC_FP64 if args.precision == "fp64" else W.WS_PREC_MIXED
+    ctx = W.Context(W.default_config(nx, ny, nz, precision=prec))
+    prims = I.sod_noise(nx, ny, nz, 20240601)
+    st = W.TorchState(nx, ny, nz)
+    st.set_prims(*prims)
+    stream = torch.cuda.current_stream()
+    sh = W.stream_handle(stream)
+    fl = (W.WsField * 5)(st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T)
+    gjob = W.job((0, 0, 0), (nx + 8, ny + 8, nz + 8))
+    ijob = W.interior_job(nx, ny, nz)
+
+    def rhs(ev=None):
+        # one full RHS: ghost fill, sound speed, three directional kernels
+        if ev is not None:
+            ev[0].record(stream)
+        W.check(L.ws_ghost_fill_periodic(ctx.handle, fl, 5, sh))
+        W.check(L.ws_sound_speed(ctx.handle, st.ws.T, st.ws.cc, gjob, sh))
+        if ev is not None:
+            ev[1].record(stream)
+        W.check(L.ws_inviscid_rhs(ctx.handle, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.cc, None, st.ws.du,
+                                  ijob, 0, sh))
+        if ev is not None:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        rhs()
+    torch.cuda.synchronize()
+    # whole-job timing
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rhs()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # per-phase timing with HIP events on the launch stream
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    for k in range(args.steps):
+        rhs(evs[k])
+    torch.cuda.synchronize()
+    prep_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
+    kern_ms = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
+
+    ms = wall / args.steps * 1e3
+    value = N / (ms * 1e-3) / 1e6
+    flops_cell, flops_dir = algo_flops()
+    achieved = flops_cell * N / (kern_ms * 1e-3) / 1e12
+
+    parity = None
+    if not args.no_parity:
+        # fast path against the reference-order anchor kernels on the same state
+        du_fast = st.interior("du").clone()
+        W.check(L.ws_ctx_set_kernel(ctx.handle, W.WS_KERNEL_ANCHOR))
+        rhs()
+        torch.cuda.synchronize()
+        du_ref = st.interior("du")
+        parity = float(torch.linalg.vector_norm(du_fast - du_ref) / torch.linalg.vector_norm(du_ref))
+        W.check(L.ws_ctx_set_kernel(ctx.handle, W.WS_KERNEL_TILED))
+        del du_fast
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "rhs_traffic_600x256x128.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_rhs")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == W.WS_PREC_FP64 else "f64-state/mixed-rhs",
+        "data": "synthetic (seeded Sod plus 1% noise, periodic)",
+        "config": {"workload": f"one inviscid RHS of the {nx}x{ny}x{nz} periodic box "
+                               "(ghost fill + sound speed + WENO7-SYMBO/Steger-Warming in x, y, z)",
+                   "grid": [nx, ny, nz], "precision": args.precision, "cells": N},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "k_rhs_tile x3 directions (one RHS)",
+                     "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
+        "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
+        "parity_rel_l2_vs_anchor": parity,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prims, nx, ny, nz, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

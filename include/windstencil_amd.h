@@ -1,0 +1,204 @@
+/*
+ * windstencil_amd.h -- C ABI of the MI355X inviscid-RHS engine.
+ *
+ * One GPU (gfx950), one process.  Every entry point is extern "C", takes plain
+ * descriptors and pointers, is stream-ordered on the hipStream_t it is given,
+ * performs no host synchronisation unless its comment says so, and returns
+ * WS_OK (0) or a negative WS_ERR_* code; ws_last_error() describes the last
+ * failure of the calling thread.
+ *
+ * Data layout (kept identical to the reference so that its buffers can be
+ * handed over unchanged):
+ *   ws_field  ghosted scalar field, element (x,y,z), x,y,z in [0, n+2*LAP):
+ *             ptr[x + pitch*(y + (ny+2*LAP)*z)]            (get_Field_LAP,
+ *             include/windstencil/cuda_utility.h:21)
+ *   ws_soa    5-variable interior state, element (x,y,z,m), x,y,z in [0,n):
+ *             ptr[x + pitch*(y + ny*(z + nz*m))]           (get_SoA,
+ *             include/windstencil/cuda_utility.h:29)
+ *   ws_job    half-open box [start, end) in ghosted coordinates
+ *             (cudaJobPackage, include/windstencil/cuda_commen.h:95-104)
+ * Pitches are in elements.  LAP (ghost depth) is 4.
+ */
+#ifndef WINDSTENCIL_AMD_H
+#define WINDSTENCIL_AMD_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WS_LAP 4
+#define WS_NVARS 5
+
+/* error codes */
+#define WS_OK 0
+#define WS_ERR_INVALID (-1)      /* bad argument (null pointer, bad size, bad box) */
+#define WS_ERR_UNSUPPORTED (-2)  /* valid request this build does not implement */
+#define WS_ERR_HIP (-3)          /* a HIP runtime call failed */
+#define WS_ERR_NOMEM (-4)        /* device allocation failed */
+
+/* precision of the RHS arithmetic */
+#define WS_PREC_FP64 0
+#define WS_PREC_MIXED 1
+
+/* RHS kernel family */
+#define WS_KERNEL_TILED 0        /* LDS-tiled directional kernels (default, fast path) */
+#define WS_KERNEL_ANCHOR 1       /* per-direction split + per-cell WENO kernels, reference operation
+                                    order, no contraction: bitwise equal to the CPU restatement */
+
+/* Ghosted scalar field; layout of cudaField under __HIPCC__ (cuda_commen.h:48-54). */
+typedef struct ws_field {
+    double *ptr;
+    uint32_t pitch;
+    uint32_t dummy0;
+} ws_field;
+
+/* Interior five-variable state; layout of cudaSoA under __HIPCC__ (cuda_commen.h:64-71). */
+typedef struct ws_soa {
+    double *ptr;
+    uint32_t pitch;
+    uint32_t length_y;
+    uint32_t length_z;
+    uint32_t dummy0;
+} ws_soa;
+
+typedef struct ws_dim3 {
+    uint32_t x, y, z;
+} ws_dim3;
+
+/* Half-open box; layout of cudaJobPackage (two dim3). */
+typedef struct ws_job {
+    ws_dim3 start;
+    ws_dim3 end;
+} ws_job;
+
+/* Per-cell metric terms (all ghosted fields): Akx,Aky,Akz (x), Aix,Aiy,Aiz (y),
+ * Asx,Asy,Asz (z) and the Jacobian factor Ajac.  The reference keeps them in
+ * the globals pAkx_d .. pAjac_d (src/runtime/OCFD_init.cpp:412-424). */
+typedef struct ws_metric_fields {
+    ws_field A[9];
+    ws_field ajac;
+} ws_metric_fields;
+
+/* Run configuration: what the reference keeps in host globals and __constant__
+ * symbols (src/runtime/parameters_d.cpp:16-31, OCFD_init.cpp:29-31,379-396). */
+typedef struct ws_config {
+    int32_t nx, ny, nz;          /* interior extents of the block (one GPU: the domain) */
+    int32_t periodic[3];         /* 1 = periodic in that direction (the only mode of this build) */
+    double gamma;                /* Gamma */
+    double ama;                  /* Ama (reference Mach number) */
+    double epsl_sw;              /* EPSL_SW, Steger-Warming smoothing */
+    double dt;                   /* DT, time step used by the reference-named wrappers */
+    double hx, hy, hz;           /* grid spacings (the reference sets 1/(N_global-1)) */
+    double metric[10];           /* constant metrics Akx..Asz, Ajac, used when no metric fields are given */
+    int32_t precision;           /* WS_PREC_FP64 or WS_PREC_MIXED */
+    int32_t kernel;              /* WS_KERNEL_TILED or WS_KERNEL_ANCHOR */
+    int32_t bc_planes;           /* 1: OCFD_bc re-derivation of the boundary planes after each stage */
+    int32_t reserved;
+} ws_config;
+
+typedef struct ws_ctx ws_ctx;
+
+/* Solver state: the device buffers the reference allocates in
+ * opencfd_mem_init_dev (src/runtime/OCFD_init.cpp:411-446). */
+typedef struct ws_state {
+    ws_soa f, fn, du;                       /* pf_d, pfn_d, pdu_d */
+    ws_field d, u, v, w, T, P, cc;          /* pd_d .. pP_d, pcc_d */
+} ws_state;
+
+/* ---- library ------------------------------------------------------------ */
+const char *ws_version(void);
+const char *ws_error_string(int code);
+const char *ws_last_error(void);
+void ws_config_default(ws_config *cfg, int32_t nx, int32_t ny, int32_t nz);
+
+/* ---- context (no counterpart: the reference uses globals) ---------------- */
+int ws_ctx_create(const ws_config *cfg, ws_ctx **out);
+int ws_ctx_destroy(ws_ctx *ctx);
+int ws_ctx_get_config(const ws_ctx *ctx, ws_config *out);
+int ws_ctx_set_precision(ws_ctx *ctx, int32_t precision);
+int ws_ctx_set_kernel(ws_ctx *ctx, int32_t kernel);
+
+/* Allocate / free the solver state with the reference's extents
+ * (replaces opencfd_mem_init_dev, src/runtime/OCFD_init.cpp:411-446). */
+int ws_state_alloc(ws_ctx *ctx, ws_state *out);
+int ws_state_free(ws_ctx *ctx, ws_state *st);
+
+/* ---- hot path ------------------------------------------------------------ */
+
+/* Inviscid RHS of the interior box `job` from ghosted primitives d,u,v,w and sound
+ * speed cc into du: du = RHS (accumulate = 0) or du += RHS (accumulate = 1).
+ * Replaces split_Jac3d_Stager_Warming_ker + split_Jac3d_Stager_Warming_ker_boundary
+ * (include/windstencil/OCFD_split.h:15-38) as launched by du_invis_Jacobian3d_x
+ * (src/solver/OCFD_NS_Jacobian3d.cpp:240-284).  Ghosts of d,u,v,w,cc (LAP deep along
+ * each direction, faces only) must be filled.  metrics == NULL: constant metrics from
+ * the config.  job must satisfy LAP <= start < end <= n+LAP in every direction. */
+int ws_inviscid_rhs(ws_ctx *ctx, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                    const ws_metric_fields *metrics, ws_soa du, ws_job job, int32_t accumulate,
+                    hipStream_t stream);
+
+/* Sound speed cc = sqrt(T)/Ma over the box (ghosted coordinates).
+ * Replaces du_invis_Jacobian3d_init / sound_speed_kernel
+ * (src/solver/OCFD_NS_Jacobian3d.cpp:24-33, src/kernels/commen_kernel.cpp:145-154). */
+int ws_sound_speed(ws_ctx *ctx, ws_field T, ws_field cc, ws_job job, hipStream_t stream);
+
+/* On-device periodic LAP-deep face halo of n fields: x, then y, then z, each over the
+ * interior of the other two axes; edges and corners untouched.  Replaces
+ * exchange_boundary_xyz_packed_dev on one rank with periodic neighbours
+ * (src/mpi/OCFD_mpi_dev.cpp:123-128,187-290; src/mpi/OCFD_mpi.c:171-197). */
+int ws_ghost_fill_periodic(ws_ctx *ctx, const ws_field *fields, int32_t nfields, hipStream_t stream);
+
+/* Full RHS of one stage from the state's primitives: periodic ghost fill of
+ * d,u,v,w,T, sound speed over the ghosted box, inviscid RHS into du (overwritten).
+ * Replaces du_comput's inviscid part (src/solver/OCFD_Stream.cpp:27-52,185-227). */
+int ws_du_comput(ws_ctx *ctx, const ws_state *st, const ws_metric_fields *metrics, hipStream_t stream);
+
+/* ---- state helpers ------------------------------------------------------- */
+
+/* Conserved -> primitive on the interior (get_duvwT, src/kernels/commen_kernel.cpp:78-119).
+ * P may have a null ptr. */
+int ws_cons_to_pri(ws_ctx *ctx, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w,
+                   ws_field T, ws_field P, hipStream_t stream);
+
+/* Primitive -> conserved over the interior box `job` given in interior coordinates
+ * (pri_to_cons_kernel, src/kernels/commen_kernel.cpp:32-66). */
+int ws_pri_to_cons(ws_ctx *ctx, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w,
+                   ws_field T, ws_job job, hipStream_t stream);
+
+/* OCFD_bc for periodic boxes: conserved variables of the six boundary planes from the
+ * primitives (src/boundary/OCFD_boundary.cpp:38-89). */
+int ws_bc_planes(ws_ctx *ctx, const ws_state *st, hipStream_t stream);
+
+/* SSP-RK3 stage krk (1..3) over the interior (OCFD_time_advance_ker1..3,
+ * src/solver/OCFD_time.cpp:15-65).  f_lap.ptr may be null (stage 3 then skips it). */
+int ws_time_advance(ws_ctx *ctx, int32_t krk, ws_soa f, ws_soa fn, ws_soa du, ws_soa f_lap,
+                    double dt, hipStream_t stream);
+
+/* Fill a pitched box with a value (cuda_mem_value_init_warp,
+ * src/kernels/commen_kernel.cpp:12-29). */
+int ws_mem_value_init(double value, double *ptr, uint32_t pitch, uint32_t size_x, uint32_t size_y,
+                      uint32_t size_z, hipStream_t stream);
+
+/* CFL step dt = cfl*min(h)/max_cells max_dir(|u_dir| + c) written to *dt_dev (a device
+ * double).  No counterpart: the reference uses a fixed DT. */
+int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipStream_t stream);
+
+/* ---- time integration ---------------------------------------------------- */
+
+/* One SSP-RK3 step (the reference loop body, src/solver/OCFD_NS_Solver.cpp:64-90, with
+ * all three stages): swap f/fn, then per stage du_comput, time advance, get_duvwT and
+ * (if enabled) OCFD_bc.  dt_dev: device double holding dt, or NULL to use cfg.dt. */
+int ws_rk3_step(ws_ctx *ctx, ws_state *st, const double *dt_dev, hipStream_t stream);
+
+/* nsteps RK3 steps.  cfl > 0: dt recomputed on the device at the start of every step
+ * with ws_cfl_dt; cfl <= 0: cfg.dt.  use_graph = 1 captures one step in a hipGraph
+ * and replays it.  Synchronises the stream before returning. */
+int ws_rk3_run(ws_ctx *ctx, ws_state *st, int32_t nsteps, double cfl, int32_t use_graph,
+               hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,177 @@
+"""GPU parity of the state helpers, the SSP-RK3 driver (eager and graph-replayed)
+and the reference-named entry points, against the CPU oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import windstencil_amd as W
+from windstencil_amd import LAP
+from tests.gpu_util import log, make_ctx, upload_state
+from tests.helpers import ghosted, random_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _ghosted_np(t):
+    return t.cpu().numpy()[:, :, :]
+
+
+def test_ghost_fill_sound_speed_and_conversions_bitwise(orc, ws):
+    import torch
+    nx, ny, nz = 29, 18, 13
+    L = W.lib()
+    s = W.stream_handle()
+    ctx = make_ctx(nx, ny, nz)
+    prims = random_state(nx, ny, nz, 4, "rough")
+    st = upload_state(prims)
+    p = orc.params(nx, ny, nz)
+    # primitive -> conserved over the interior
+    W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                             W.job((0, 0, 0), (nx, ny, nz)), s))
+    _sync()
+    g = [ghosted(a) for a in prims]
+    f_ref = orc.pri_to_cons(p, *g)
+    assert np.array_equal(st.interior("f").cpu().numpy(), f_ref)
+    # conserved -> primitive
+    st2 = W.TorchState(nx, ny, nz, pad=3)
+    st2.t["f"][:, :, :, :nx].copy_(torch.from_numpy(f_ref))
+    W.check(L.ws_cons_to_pri(ctx.handle, st2.ws.f, st2.ws.d, st2.ws.u, st2.ws.v, st2.ws.w, st2.ws.T,
+                             st2.ws.P, s))
+    _sync()
+    refs = orc.cons_to_pri(p, f_ref)
+    for k, r in zip(("d", "u", "v", "w", "T", "P"), refs):
+        assert np.array_equal(st2.interior(k).cpu().numpy(), r[LAP:-LAP, LAP:-LAP, LAP:-LAP]), k
+    # ghost fill (faces only) and sound speed over the ghosted box
+    fl = (W.WsField * 1)(st2.ws.T)
+    W.check(L.ws_ghost_fill_periodic(ctx.handle, fl, 1, s))
+    W.check(L.ws_sound_speed(ctx.handle, st2.ws.T, st2.ws.cc,
+                             W.job((0, 0, 0), (nx + 8, ny + 8, nz + 8)), s))
+    _sync()
+    T_ref = refs[4].copy()
+    orc.ghost_fill_periodic(p, T_ref)
+    T_gpu = st2.t["T"].cpu().numpy()[:, :, :nx + 8]
+    assert np.array_equal(T_gpu, T_ref)
+    cc_ref = orc.sound_speed(p, T_ref)
+    assert np.array_equal(st2.t["cc"].cpu().numpy()[:, :, :nx + 8], cc_ref)
+
+
+def test_time_advance_stages_bitwise(orc, ws):
+    import torch
+    nx, ny, nz = 20, 12, 9
+    L = W.lib()
+    s = W.stream_handle()
+    ctx = make_ctx(nx, ny, nz)
+    p = orc.params(nx, ny, nz)
+    rng = np.random.default_rng(2)
+    fn = rng.random((5, nz, ny, nx))
+    du = rng.standard_normal((5, nz, ny, nx))
+    f = rng.random((5, nz, ny, nx))
+    st = W.TorchState(nx, ny, nz, pad=2)
+    for k, a in (("fn", fn), ("du", du), ("f", f)):
+        st.interior(k).copy_(torch.from_numpy(a))
+    fc = f.copy()
+    for krk in (1, 2, 3):
+        W.check(L.ws_time_advance(ctx.handle, krk, st.ws.f, st.ws.fn, st.ws.du, W.WsSoa(), 0.013, s))
+        orc.time_advance(p, krk, fc, fn, du, 0.013)
+        _sync()
+        assert np.array_equal(st.interior("f").cpu().numpy(), fc), krk
+
+
+def _gpu_run(prims, nsteps, kernel, graph, cfl=0.5):
+    nz, ny, nx = prims[0].shape
+    L = W.lib()
+    s = W.stream_handle()
+    ctx = make_ctx(nx, ny, nz, kernel=kernel)
+    st = upload_state(prims)
+    W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                             W.job((0, 0, 0), (nx, ny, nz)), s))
+    W.check(L.ws_rk3_run(ctx.handle, ctypes.byref(st.ws), nsteps, cfl, 1 if graph else 0, s), "rk3 run")
+    st.sync_from_ws()
+    _sync()
+    return st.interior("f").cpu().numpy(), st
+
+
+def _oracle_run(orc, prims, nsteps, cfl=0.5):
+    nz, ny, nx = prims[0].shape
+    p = orc.params(nx, ny, nz)
+    g = [ghosted(a) for a in prims]
+    f = orc.pri_to_cons(p, *g)
+    orc.rk3_run(p, nsteps, -cfl, f, *g)
+    return f
+
+
+def test_rk3_cpu_workload_anchor_bitwise(orc, ws):
+    """The CPU-path workload: 64x32x16, Mach 2 with 8 seeded blobs, 10 SSP-RK3
+    steps at CFL 0.5 (dt recomputed on the device every step)."""
+    from windstencil_amd import initial as I
+    prims = I.freestream_blobs(64, 32, 16, 1)
+    log("rk3 anchor 64x32x16 x10")
+    f_gpu, _ = _gpu_run(prims, 10, W.WS_KERNEL_ANCHOR, graph=False)
+    f_ref = _oracle_run(orc, prims, 10)
+    assert np.array_equal(f_gpu, f_ref), rel_l2(f_gpu, f_ref)
+
+
+@pytest.mark.parametrize("kind", ["blobs", "vortex", "shock"])
+def test_rk3_tiled_matches_oracle(orc, ws, kind):
+    prims = random_state(48, 32, 16, 11, kind)
+    log(f"rk3 tiled {kind} x10")
+    f_gpu, _ = _gpu_run(prims, 10, W.WS_KERNEL_TILED, graph=False)
+    f_ref = _oracle_run(orc, prims, 10)
+    err = rel_l2(f_gpu, f_ref)
+    log(f"  rel L2 {err:.3e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("nsteps", [2, 3, 4, 5])
+def test_graph_replay_equals_eager(ws, nsteps):
+    """Every replay of the captured step (steps 1..n-1) against the eager path."""
+    prims = random_state(32, 16, 16, 9, "shock")
+    log(f"graph vs eager {nsteps}")
+    a, _ = _gpu_run(prims, nsteps, W.WS_KERNEL_TILED, graph=True)
+    b, _ = _gpu_run(prims, nsteps, W.WS_KERNEL_TILED, graph=False)
+    assert np.array_equal(a, b)
+
+
+def test_reference_named_entry_points(ws):
+    """The solver-facing names (du_comput, OCFD_time_advance, get_duvwT, OCFD_bc)
+    reproduce one SSP-RK3 stage of the C API."""
+    import torch
+    nx, ny, nz = 30, 16, 12
+    L = W.lib()
+    s = W.stream_handle()
+    prims = random_state(nx, ny, nz, 6, "blobs")
+    ctx = make_ctx(nx, ny, nz, dt=2e-4)
+    ref = upload_state(prims)
+    W.check(L.ws_pri_to_cons(ctx.handle, ref.ws.f, ref.ws.d, ref.ws.u, ref.ws.v, ref.ws.w, ref.ws.T,
+                             W.job((0, 0, 0), (nx, ny, nz)), s))
+    other = upload_state(prims)
+    other.t["f"].copy_(ref.t["f"])
+    for st in (ref, other):
+        st.t["fn"].copy_(st.t["f"])
+    # C API stage 1
+    W.check(L.ws_du_comput(ctx.handle, ctypes.byref(ref.ws), None, s))
+    W.check(L.ws_time_advance(ctx.handle, 1, ref.ws.f, ref.ws.fn, ref.ws.du, W.WsSoa(), 2e-4, s))
+    W.check(L.ws_cons_to_pri(ctx.handle, ref.ws.f, ref.ws.d, ref.ws.u, ref.ws.v, ref.ws.w, ref.ws.T,
+                             ref.ws.P, s))
+    W.check(L.ws_bc_planes(ctx.handle, ctypes.byref(ref.ws), s))
+    # reference names
+    L.ws_ref_set_abort_on_error(0)
+    W.check(L.ws_ref_bind(ctx.handle, ctypes.byref(other.ws), None, s))
+    L.du_comput(1)
+    L.OCFD_time_advance(1)
+    L.get_duvwT()
+    L.OCFD_bc()
+    assert L.ws_ref_last_status() == 0
+    _sync()
+    for k in ("f", "du", "d", "u", "v", "w", "T"):
+        assert torch.equal(ref.t[k], other.t[k]), k
+    # an error is recorded, not aborted, after ws_ref_set_abort_on_error(0)
+    L.OCFD_time_advance(7)
+    assert L.ws_ref_last_status() == -1
+    L.ws_ref_set_abort_on_error(1)

@@ -1,0 +1,480 @@
+// capi.cpp -- the extern "C" entry points: context, validation, dispatch, RK3 driver.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "ws_internal.h"
+
+struct ws_ctx : public ws::Ctx {};
+
+namespace ws {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_error(hipError_t e, const char *what)
+{
+    return set_error(WS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+KParams make_kparams(const ws_config &c)
+{
+    KParams k{};
+    k.nx = c.nx; k.ny = c.ny; k.nz = c.nz;
+    k.nyg = c.ny + 2 * LAP; k.nzg = c.nz + 2 * LAP;
+    k.gamma = c.gamma;
+    k.c1 = 2.0 * (c.gamma - 1.0);                          // OCFD_init.cpp:385
+    k.c3 = (3.0 - c.gamma) / (2.0 * (c.gamma - 1.0));      // OCFD_init.cpp:386
+    k.half_rcp_gamma = 0.5 / c.gamma;
+    k.gm1 = c.gamma - 1.0;
+    k.eps2 = c.epsl_sw * c.epsl_sw;
+    k.ama = c.ama; k.rcp_ama = 1.0 / c.ama;
+    k.cv = 1.0 / (c.gamma * (c.gamma - 1.0) * c.ama * c.ama);  // OCFD_init.cpp:380
+    const double h[3] = {c.hx, c.hy, c.hz};
+    for (int i = 0; i < 3; ++i) { k.h[i] = h[i]; k.rh[i] = 1.0 / h[i]; }
+    for (int i = 0; i < 9; ++i) k.A[i] = c.metric[i];
+    for (int dir = 0; dir < 3; ++dir) {
+        const double a1 = c.metric[3 * dir], a2 = c.metric[3 * dir + 1], a3 = c.metric[3 * dir + 2];
+        const double sn = std::sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+        k.Anorm[dir] = sn;
+        k.Ak[3 * dir] = a1 / sn; k.Ak[3 * dir + 1] = a2 / sn; k.Ak[3 * dir + 2] = a3 / sn;
+    }
+    k.ajac = c.metric[9];
+    return k;
+}
+
+}  // namespace ws
+
+using namespace ws;
+
+namespace {
+
+bool field_ok(const ws_field &f) { return f.ptr != nullptr && f.pitch > 0; }
+
+int check_field(const ws_ctx *c, const ws_field &f, const char *name)
+{
+    if (!field_ok(f)) return set_error(WS_ERR_INVALID, std::string(name) + ": null field");
+    if (f.pitch < (uint32_t)(c->cfg.nx + 2 * LAP))
+        return set_error(WS_ERR_INVALID, std::string(name) + ": pitch smaller than nx+2*LAP");
+    return WS_OK;
+}
+
+int check_soa(const ws_ctx *c, const ws_soa &s, const char *name)
+{
+    if (!s.ptr || s.pitch == 0) return set_error(WS_ERR_INVALID, std::string(name) + ": null state");
+    if (s.pitch < (uint32_t)c->cfg.nx) return set_error(WS_ERR_INVALID, std::string(name) + ": pitch smaller than nx");
+    return WS_OK;
+}
+
+int check_interior_job(const ws_ctx *c, const ws_job &j)
+{
+    const uint32_t n[3] = {(uint32_t)c->cfg.nx, (uint32_t)c->cfg.ny, (uint32_t)c->cfg.nz};
+    const uint32_t s[3] = {j.start.x, j.start.y, j.start.z};
+    const uint32_t e[3] = {j.end.x, j.end.y, j.end.z};
+    for (int i = 0; i < 3; ++i)
+        if (s[i] < (uint32_t)LAP || e[i] <= s[i] || e[i] > n[i] + LAP)
+            return set_error(WS_ERR_INVALID, "job box must satisfy LAP <= start < end <= n+LAP");
+    return WS_OK;
+}
+
+int check_metrics(const ws_ctx *c, const ws_metric_fields *m)
+{
+    if (!m) return WS_OK;
+    for (int i = 0; i < 9; ++i) {
+        int r = check_field(c, m->A[i], "metric");
+        if (r) return r;
+    }
+    return check_field(c, m->ajac, "ajac");
+}
+
+ws_job interior_job(const ws_ctx *c)
+{
+    ws_job j;
+    j.start = {(uint32_t)LAP, (uint32_t)LAP, (uint32_t)LAP};
+    j.end = {(uint32_t)(c->cfg.nx + LAP), (uint32_t)(c->cfg.ny + LAP), (uint32_t)(c->cfg.nz + LAP)};
+    return j;
+}
+
+ws_job ghosted_job(const ws_ctx *c)
+{
+    ws_job j;
+    j.start = {0, 0, 0};
+    j.end = {(uint32_t)(c->cfg.nx + 2 * LAP), (uint32_t)(c->cfg.ny + 2 * LAP), (uint32_t)(c->cfg.nz + 2 * LAP)};
+    return j;
+}
+
+int validate_config(const ws_config *cfg)
+{
+    if (!cfg) return set_error(WS_ERR_INVALID, "null config");
+    if (cfg->nx < LAP || cfg->ny < LAP || cfg->nz < LAP)
+        return set_error(WS_ERR_INVALID, "every extent must be at least LAP = 4 (periodic halo)");
+    if ((long)cfg->nx * cfg->ny * cfg->nz > (1L << 33))
+        return set_error(WS_ERR_INVALID, "grid too large");
+    for (int i = 0; i < 3; ++i)
+        if (cfg->periodic[i] != 1)
+            return set_error(WS_ERR_UNSUPPORTED, "only periodic boundaries are implemented");
+    if (!(cfg->gamma > 1.0) || !(cfg->ama > 0.0) || !(cfg->epsl_sw >= 0.0))
+        return set_error(WS_ERR_INVALID, "gamma must exceed 1, Ma must be positive, epsl_sw non-negative");
+    if (!(cfg->hx > 0.0) || !(cfg->hy > 0.0) || !(cfg->hz > 0.0))
+        return set_error(WS_ERR_INVALID, "grid spacings must be positive");
+    for (int dir = 0; dir < 3; ++dir) {
+        const double *a = cfg->metric + 3 * dir;
+        if (!(a[0] * a[0] + a[1] * a[1] + a[2] * a[2] > 0.0))
+            return set_error(WS_ERR_INVALID, "constant metric of a direction has zero norm");
+    }
+    if (cfg->precision != WS_PREC_FP64 && cfg->precision != WS_PREC_MIXED)
+        return set_error(WS_ERR_INVALID, "unknown precision");
+    if (cfg->kernel != WS_KERNEL_TILED && cfg->kernel != WS_KERNEL_ANCHOR)
+        return set_error(WS_ERR_INVALID, "unknown kernel family");
+    return WS_OK;
+}
+
+int alloc_field(ws_field *f, int nx, int ny, int nz)
+{
+    const uint32_t pitch = (uint32_t)((nx + 2 * LAP + 31) / 32 * 32);   // 256-byte rows
+    const size_t n = (size_t)pitch * (ny + 2 * LAP) * (nz + 2 * LAP);
+    hipError_t e = hipMalloc(&f->ptr, n * sizeof(double));
+    if (e != hipSuccess) { f->ptr = nullptr; return set_error(WS_ERR_NOMEM, "field allocation failed"); }
+    e = hipMemset(f->ptr, 0, n * sizeof(double));
+    if (e != hipSuccess) return hip_error(e, "field clear");
+    f->pitch = pitch;
+    f->dummy0 = 0;
+    return WS_OK;
+}
+
+int alloc_soa(ws_soa *s, int nx, int ny, int nz)
+{
+    const uint32_t pitch = (uint32_t)((nx + 31) / 32 * 32);
+    const size_t n = (size_t)pitch * ny * nz * 5;
+    hipError_t e = hipMalloc(&s->ptr, n * sizeof(double));
+    if (e != hipSuccess) { s->ptr = nullptr; return set_error(WS_ERR_NOMEM, "state allocation failed"); }
+    e = hipMemset(s->ptr, 0, n * sizeof(double));
+    if (e != hipSuccess) return hip_error(e, "state clear");
+    s->pitch = pitch;
+    s->length_y = (uint32_t)ny;
+    s->length_z = (uint32_t)nz;
+    s->dummy0 = 0;
+    return WS_OK;
+}
+
+int rhs_dispatch(ws_ctx *c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                 const ws_metric_fields *m, ws_soa du, ws_job job, int acc, hipStream_t s)
+{
+    if (c->cfg.kernel == WS_KERNEL_ANCHOR) return launch_rhs_anchor(*c, d, u, v, w, cc, m, du, job, acc, s);
+    if (c->cfg.precision == WS_PREC_MIXED) return launch_rhs_mixed(*c, d, u, v, w, cc, m, du, job, acc, s);
+    return launch_rhs_tiled(*c, d, u, v, w, cc, m, du, job, acc, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ws_version(void) { return "windstencil_amd 0.1 (gfx950)"; }
+
+const char *ws_error_string(int code)
+{
+    switch (code) {
+    case WS_OK: return "ok";
+    case WS_ERR_INVALID: return "invalid argument";
+    case WS_ERR_UNSUPPORTED: return "unsupported";
+    case WS_ERR_HIP: return "HIP runtime error";
+    case WS_ERR_NOMEM: return "out of device memory";
+    default: return "unknown error";
+    }
+}
+
+const char *ws_last_error(void) { return g_last_error.c_str(); }
+
+void ws_config_default(ws_config *cfg, int32_t nx, int32_t ny, int32_t nz)
+{
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->nx = nx; cfg->ny = ny; cfg->nz = nz;
+    cfg->periodic[0] = cfg->periodic[1] = cfg->periodic[2] = 1;
+    cfg->gamma = 1.4;
+    cfg->ama = 1.0;
+    cfg->epsl_sw = 1e-3;
+    cfg->dt = 1e-4;
+    cfg->hx = 1.0 / (nx - 1.0);
+    cfg->hy = 1.0 / (ny - 1.0);
+    cfg->hz = 1.0 / (nz - 1.0);
+    // Cartesian metrics: Akx = Aiy = Asz = Ajac = 1
+    cfg->metric[0] = 1.0; cfg->metric[4] = 1.0; cfg->metric[8] = 1.0; cfg->metric[9] = 1.0;
+    cfg->precision = WS_PREC_FP64;
+    cfg->kernel = WS_KERNEL_TILED;
+    cfg->bc_planes = 1;
+}
+
+int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
+{
+    if (!out) return set_error(WS_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    int r = validate_config(cfg);
+    if (r) return r;
+    ws_ctx *c = new (std::nothrow) ws_ctx();
+    if (!c) return set_error(WS_ERR_NOMEM, "host allocation failed");
+    c->cfg = *cfg;
+    c->kp = make_kparams(*cfg);
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) { delete c; return hip_error(e, "hipGetDevice"); }
+    c->cfl_blocks = 1024;
+    e = hipMalloc(&c->cfl_partial, sizeof(double) * c->cfl_blocks);
+    if (e != hipSuccess) { delete c; return hip_error(e, "CFL scratch allocation"); }
+    e = hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { hipFree(c->cfl_partial); delete c; return hip_error(e, "capture stream"); }
+    *out = c;
+    return WS_OK;
+}
+
+int ws_ctx_destroy(ws_ctx *c)
+{
+    if (!c) return WS_OK;
+    if (c->capture_stream) hipStreamDestroy(c->capture_stream);
+    if (c->cfl_partial) hipFree(c->cfl_partial);
+    delete c;
+    return WS_OK;
+}
+
+int ws_ctx_get_config(const ws_ctx *c, ws_config *out)
+{
+    if (!c || !out) return set_error(WS_ERR_INVALID, "null argument");
+    *out = c->cfg;
+    return WS_OK;
+}
+
+int ws_ctx_set_precision(ws_ctx *c, int32_t precision)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (precision != WS_PREC_FP64 && precision != WS_PREC_MIXED) return set_error(WS_ERR_INVALID, "unknown precision");
+    c->cfg.precision = precision;
+    return WS_OK;
+}
+
+int ws_ctx_set_kernel(ws_ctx *c, int32_t kernel)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (kernel != WS_KERNEL_TILED && kernel != WS_KERNEL_ANCHOR) return set_error(WS_ERR_INVALID, "unknown kernel");
+    c->cfg.kernel = kernel;
+    return WS_OK;
+}
+
+int ws_state_alloc(ws_ctx *c, ws_state *st)
+{
+    if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
+    std::memset(st, 0, sizeof(*st));
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    int r = WS_OK;
+    ws_soa *soas[3] = {&st->f, &st->fn, &st->du};
+    for (ws_soa *s : soas)
+        if (!r) r = alloc_soa(s, nx, ny, nz);
+    ws_field *fields[7] = {&st->d, &st->u, &st->v, &st->w, &st->T, &st->P, &st->cc};
+    for (ws_field *f : fields)
+        if (!r) r = alloc_field(f, nx, ny, nz);
+    if (r) ws_state_free(c, st);
+    return r;
+}
+
+int ws_state_free(ws_ctx *c, ws_state *st)
+{
+    (void)c;
+    if (!st) return WS_OK;
+    ws_soa *soas[3] = {&st->f, &st->fn, &st->du};
+    for (ws_soa *s : soas)
+        if (s->ptr) { hipFree(s->ptr); s->ptr = nullptr; }
+    ws_field *fields[7] = {&st->d, &st->u, &st->v, &st->w, &st->T, &st->P, &st->cc};
+    for (ws_field *f : fields)
+        if (f->ptr) { hipFree(f->ptr); f->ptr = nullptr; }
+    return WS_OK;
+}
+
+int ws_inviscid_rhs(ws_ctx *c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                    const ws_metric_fields *metrics, ws_soa du, ws_job job, int32_t accumulate, hipStream_t s)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    int r;
+    if ((r = check_field(c, d, "d")) || (r = check_field(c, u, "u")) || (r = check_field(c, v, "v")) ||
+        (r = check_field(c, w, "w")) || (r = check_field(c, cc, "cc")) || (r = check_soa(c, du, "du")) ||
+        (r = check_metrics(c, metrics)) || (r = check_interior_job(c, job)))
+        return r;
+    return rhs_dispatch(c, d, u, v, w, cc, metrics, du, job, accumulate ? 1 : 0, s);
+}
+
+int ws_sound_speed(ws_ctx *c, ws_field T, ws_field cc, ws_job job, hipStream_t s)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    int r;
+    if ((r = check_field(c, T, "T")) || (r = check_field(c, cc, "cc"))) return r;
+    const uint32_t g[3] = {(uint32_t)c->cfg.nx + 2 * LAP, (uint32_t)c->cfg.ny + 2 * LAP, (uint32_t)c->cfg.nz + 2 * LAP};
+    if (job.end.x > g[0] || job.end.y > g[1] || job.end.z > g[2] || job.start.x >= job.end.x ||
+        job.start.y >= job.end.y || job.start.z >= job.end.z)
+        return set_error(WS_ERR_INVALID, "sound speed box outside the ghosted field");
+    return launch_sound_speed(*c, T, cc, job, s);
+}
+
+int ws_ghost_fill_periodic(ws_ctx *c, const ws_field *fields, int32_t n, hipStream_t s)
+{
+    if (!c || !fields || n <= 0) return set_error(WS_ERR_INVALID, "bad field list");
+    for (int i = 0; i < n; ++i) {
+        int r = check_field(c, fields[i], "ghost fill");
+        if (r) return r;
+    }
+    return launch_ghost_fill(*c, fields, n, s);
+}
+
+int ws_du_comput(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics, hipStream_t s)
+{
+    if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
+    int r;
+    const ws_field g[5] = {st->d, st->u, st->v, st->w, st->T};
+    if ((r = ws_ghost_fill_periodic(c, g, 5, s))) return r;
+    if ((r = launch_sound_speed(*c, st->T, st->cc, ghosted_job(c), s))) return r;
+    return ws_inviscid_rhs(c, st->d, st->u, st->v, st->w, st->cc, metrics, st->du, interior_job(c), 0, s);
+}
+
+int ws_cons_to_pri(ws_ctx *c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T, ws_field P,
+                   hipStream_t s)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    int r;
+    if ((r = check_soa(c, f, "f")) || (r = check_field(c, d, "d")) || (r = check_field(c, u, "u")) ||
+        (r = check_field(c, v, "v")) || (r = check_field(c, w, "w")) || (r = check_field(c, T, "T")))
+        return r;
+    if (P.ptr && (r = check_field(c, P, "P"))) return r;
+    return launch_cons_to_pri(*c, f, d, u, v, w, T, P, s);
+}
+
+int ws_pri_to_cons(ws_ctx *c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T, ws_job job,
+                   hipStream_t s)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    int r;
+    if ((r = check_soa(c, f, "f")) || (r = check_field(c, d, "d")) || (r = check_field(c, u, "u")) ||
+        (r = check_field(c, v, "v")) || (r = check_field(c, w, "w")) || (r = check_field(c, T, "T")))
+        return r;
+    if (job.end.x > (uint32_t)c->cfg.nx || job.end.y > (uint32_t)c->cfg.ny || job.end.z > (uint32_t)c->cfg.nz ||
+        job.start.x >= job.end.x || job.start.y >= job.end.y || job.start.z >= job.end.z)
+        return set_error(WS_ERR_INVALID, "pri_to_cons box outside the interior");
+    return launch_pri_to_cons(*c, f, d, u, v, w, T, job, s);
+}
+
+int ws_bc_planes(ws_ctx *c, const ws_state *st, hipStream_t s)
+{
+    if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
+    const uint32_t nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    // x = 0, x = nx-1, y = 0, y = ny-1, z = 0, z = nz-1 (OCFD_boundary.cpp:38-89)
+    const ws_job planes[6] = {
+        {{0, 0, 0}, {1, ny, nz}},       {{nx - 1, 0, 0}, {nx, ny, nz}},
+        {{0, 0, 0}, {nx, 1, nz}},       {{0, ny - 1, 0}, {nx, ny, nz}},
+        {{0, 0, 0}, {nx, ny, 1}},       {{0, 0, nz - 1}, {nx, ny, nz}},
+    };
+    for (const ws_job &j : planes) {
+        int r = launch_pri_to_cons(*c, st->f, st->d, st->u, st->v, st->w, st->T, j, s);
+        if (r) return r;
+    }
+    return WS_OK;
+}
+
+int ws_time_advance(ws_ctx *c, int32_t krk, ws_soa f, ws_soa fn, ws_soa du, ws_soa f_lap, double dt, hipStream_t s)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (krk < 1 || krk > 3) return set_error(WS_ERR_INVALID, "krk must be 1, 2 or 3");
+    int r;
+    if ((r = check_soa(c, f, "f")) || (r = check_soa(c, fn, "fn")) || (r = check_soa(c, du, "du"))) return r;
+    return launch_time_advance(*c, krk, f, fn, du, f_lap, dt, nullptr, s);
+}
+
+int ws_mem_value_init(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy, uint32_t sz,
+                      hipStream_t s)
+{
+    if (!ptr || pitch < sx) return set_error(WS_ERR_INVALID, "bad fill target");
+    if (sx == 0 || sy == 0 || sz == 0) return WS_OK;
+    return launch_fill(value, ptr, pitch, sx, sy, sz, s);
+}
+
+int ws_cfl_dt(ws_ctx *c, const ws_state *st, double cfl, double *dt_dev, hipStream_t s)
+{
+    if (!c || !st || !dt_dev || !(cfl > 0.0)) return set_error(WS_ERR_INVALID, "bad CFL request");
+    return launch_cfl(*c, *st, cfl, dt_dev, s);
+}
+
+int ws_rk3_step(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
+{
+    if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
+    // pf <-> pfn (OCFD_NS_Solver.cpp:66-74): fn now holds the state at t^n
+    std::swap(st->f, st->fn);
+    const ws_soa nolap{};
+    for (int krk = 1; krk <= 3; ++krk) {
+        int r;
+        if ((r = ws_du_comput(c, st, nullptr, s))) return r;
+        if ((r = launch_time_advance(*c, krk, st->f, st->fn, st->du, nolap, c->cfg.dt, dt_dev, s))) return r;
+        if ((r = launch_cons_to_pri(*c, st->f, st->d, st->u, st->v, st->w, st->T, st->P, s))) return r;
+        if (c->cfg.bc_planes && (r = ws_bc_planes(c, st, s))) return r;
+    }
+    return WS_OK;
+}
+
+int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t s)
+{
+    if (!c || !st || nsteps < 0) return set_error(WS_ERR_INVALID, "bad RK3 request");
+    double *dt_dev = nullptr;
+    if (cfl > 0.0) {
+        hipError_t e = hipMalloc(&dt_dev, sizeof(double));
+        if (e != hipSuccess) return hip_error(e, "dt allocation");
+    }
+    int r = WS_OK;
+    if (!use_graph) {
+        for (int n = 0; n < nsteps && !r; ++n) {
+            if (dt_dev) r = ws_cfl_dt(c, st, cfl, dt_dev, s);
+            if (!r) r = ws_rk3_step(c, st, dt_dev, s);
+        }
+    } else {
+        // Step 0 runs eagerly (it performs any lazy allocation); later steps replay one
+        // captured graph per parity of the f/fn swap, since a step ends with f and fn exchanged.
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
+        for (int n = 0; n < nsteps && !r; ++n) {
+            if (n == 0) {
+                if (dt_dev) r = ws_cfl_dt(c, st, cfl, dt_dev, s);
+                if (!r) r = ws_rk3_step(c, st, dt_dev, s);
+                continue;
+            }
+            const int p = n & 1;
+            if (!exec[p]) {
+                ws_state probe = *st;
+                hipGraph_t g = nullptr;
+                hipStream_t cs = c->capture_stream;
+                hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) { r = hip_error(e, "begin capture"); break; }
+                if (dt_dev) r = ws_cfl_dt(c, &probe, cfl, dt_dev, cs);
+                if (!r) r = ws_rk3_step(c, &probe, dt_dev, cs);
+                e = hipStreamEndCapture(cs, &g);
+                if (r) { if (g) hipGraphDestroy(g); break; }
+                if (e != hipSuccess) { r = hip_error(e, "end capture"); break; }
+                e = hipGraphInstantiate(&exec[p], g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (e != hipSuccess) { r = hip_error(e, "graph instantiate"); break; }
+            }
+            hipError_t e = hipGraphLaunch(exec[p], s);
+            if (e != hipSuccess) { r = hip_error(e, "graph launch"); break; }
+            std::swap(st->f, st->fn);
+        }
+        for (int p = 0; p < 2; ++p)
+            if (exec[p]) {
+                hipStreamSynchronize(s);
+                hipGraphExecDestroy(exec[p]);
+            }
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (!r && e != hipSuccess) r = hip_error(e, "RK3 run");
+    if (dt_dev) hipFree(dt_dev);
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,291 @@
+// k_aux.hip -- memory-bound helpers around the RHS: ghost fill, sound speed,
+// primitive/conserved conversion, SSP-RK3 stage updates, box fill, CFL step.
+// Their arithmetic follows the reference's expressions with contraction off, so
+// the solver state stays bit-identical to the CPU restatement between RHS calls.
+#include <hip/hip_runtime.h>
+
+#include "ws_device.h"
+
+namespace ws {
+
+namespace {
+
+constexpr int NT = 256;
+
+struct FieldList {
+    ws_field f[8];
+};
+
+// Periodic LAP-deep face halo.  Linear index over the six face slabs of one field:
+// x slabs (2*LAP*ny*nz), y slabs (2*LAP*nx*nz), z slabs (2*LAP*nx*ny).
+// Ghost points read only interior points, so the three directions need no ordering.
+__global__ void __launch_bounds__(NT) k_ghost_fill(FieldList fl, int nx, int ny, int nz, int nyg)
+{
+    const ws_field f = fl.f[blockIdx.y];
+    const long nxs = 2L * LAP * ny * nz, nys = 2L * LAP * nx * nz, nzs = 2L * LAP * nx * ny;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int dx, dy, dz, sx, sy, sz;
+    if (i < nxs) {
+        const int g = (int)(i % (2 * LAP));            // 0..3 left ghosts, 4..7 right ghosts
+        const int y = (int)((i / (2 * LAP)) % ny);
+        const int z = (int)(i / (2L * LAP * ny));
+        dy = sy = y + LAP; dz = sz = z + LAP;
+        if (g < LAP) { dx = g;                sx = nx + g; }           // [0,4) <- [nx, nx+4)
+        else         { dx = nx + LAP + g - LAP; sx = g; }               // [nx+4, nx+8) <- [4, 8)
+    } else if ((i -= nxs) < nys) {
+        const int x = (int)(i % nx);
+        const int g = (int)((i / nx) % (2 * LAP));
+        const int z = (int)(i / (2L * LAP * nx));
+        dx = sx = x + LAP; dz = sz = z + LAP;
+        if (g < LAP) { dy = g;                sy = ny + g; }
+        else         { dy = ny + g;           sy = g; }
+    } else if ((i -= nys) < nzs) {
+        const int x = (int)(i % nx);
+        const int y = (int)((i / nx) % ny);
+        const int g = (int)(i / ((long)nx * ny));
+        dx = sx = x + LAP; dy = sy = y + LAP;
+        if (g < LAP) { dz = g;                sz = nz + g; }
+        else         { dz = nz + g;           sz = g; }
+    } else {
+        return;
+    }
+    f.ptr[fidx(f.pitch, nyg, dx, dy, dz)] = f.ptr[fidx(f.pitch, nyg, sx, sy, sz)];
+}
+
+__global__ void __launch_bounds__(NT) k_sound_speed(ws_field T, ws_field cc, int sx, int sy, int sz, int nx,
+                                                    int ny, int nz, int nyg, double ama)
+{
+#pragma clang fp contract(off)
+    const long total = (long)nx * ny * nz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int x = sx + (int)(i % nx);
+    const int y = sy + (int)((i / nx) % ny);
+    const int z = sz + (int)(i / ((long)nx * ny));
+    cc.ptr[fidx(cc.pitch, nyg, x, y, z)] = sqrt(T.ptr[fidx(T.pitch, nyg, x, y, z)]) / ama;
+}
+
+__global__ void __launch_bounds__(NT) k_cons_to_pri(KParams k, ws_soa f, ws_field d, ws_field u, ws_field v,
+                                                    ws_field w, ws_field T, ws_field P)
+{
+#pragma clang fp contract(off)
+    const long total = (long)k.nx * k.ny * k.nz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int x = (int)(i % k.nx);
+    const int y = (int)((i / k.nx) % k.ny);
+    const int z = (int)(i / ((long)k.nx * k.ny));
+    const double d1 = f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 0)];
+    const double u1 = f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 1)] / d1;
+    const double v1 = f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 2)] / d1;
+    const double w1 = f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 3)] / d1;
+    double T1 = f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 4)];
+    T1 = T1 - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1);
+    const int gx = x + LAP, gy = y + LAP, gz = z + LAP;
+    d.ptr[fidx(d.pitch, k.nyg, gx, gy, gz)] = d1;
+    u.ptr[fidx(u.pitch, k.nyg, gx, gy, gz)] = u1;
+    v.ptr[fidx(v.pitch, k.nyg, gx, gy, gz)] = v1;
+    w.ptr[fidx(w.pitch, k.nyg, gx, gy, gz)] = w1;
+    T.ptr[fidx(T.pitch, k.nyg, gx, gy, gz)] = T1 / (d1 * k.cv);
+    if (P.ptr) {
+        T1 = T1 / d1;
+        P.ptr[fidx(P.pitch, k.nyg, gx, gy, gz)] = T1 * (k.gamma - 1.0);
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_pri_to_cons(KParams k, ws_soa f, ws_field d, ws_field u, ws_field v,
+                                                    ws_field w, ws_field T, int sx, int sy, int sz, int bx,
+                                                    int by, int bz)
+{
+#pragma clang fp contract(off)
+    const long total = (long)bx * by * bz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int x = sx + (int)(i % bx);
+    const int y = sy + (int)((i / bx) % by);
+    const int z = sz + (int)(i / ((long)bx * by));
+    const int gx = x + LAP, gy = y + LAP, gz = z + LAP;
+    const double dd = d.ptr[fidx(d.pitch, k.nyg, gx, gy, gz)];
+    const double uu = u.ptr[fidx(u.pitch, k.nyg, gx, gy, gz)];
+    const double vv = v.ptr[fidx(v.pitch, k.nyg, gx, gy, gz)];
+    const double ww = w.ptr[fidx(w.pitch, k.nyg, gx, gy, gz)];
+    const double TT = T.ptr[fidx(T.pitch, k.nyg, gx, gy, gz)];
+    f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 0)] = dd;
+    f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 1)] = uu * dd;
+    f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 2)] = vv * dd;
+    f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 3)] = ww * dd;
+    f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, 4)] = dd * ((uu * uu + vv * vv + ww * ww) * 0.5 + k.cv * TT);
+}
+
+template <int KRK>
+__global__ void __launch_bounds__(NT) k_time_advance(KParams k, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl,
+                                                     double dt_val, const double *dt_dev)
+{
+#pragma clang fp contract(off)
+    const long total = (long)k.nx * k.ny * k.nz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double dt = dt_dev ? *dt_dev : dt_val;
+    const int x = (int)(i % k.nx);
+    const int y = (int)((i / k.nx) % k.ny);
+    const int z = (int)(i / ((long)k.nx * k.ny));
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const size_t a = sidx(f.pitch, k.ny, k.nz, x, y, z, m);
+        const double fnv = fn.ptr[sidx(fn.pitch, k.ny, k.nz, x, y, z, m)];
+        const double duv = du.ptr[sidx(du.pitch, k.ny, k.nz, x, y, z, m)];
+        double r;
+        if (KRK == 1) {
+            r = fnv + dt * duv;
+        } else if (KRK == 2) {
+            const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
+            r = t1 * fnv + t2 * (f.ptr[a] + dt * duv);
+        } else {
+            const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
+            r = t1 * fnv + t2 * (f.ptr[a] + dt * duv);
+        }
+        f.ptr[a] = r;
+        if (KRK == 3 && fl.ptr)
+            fl.ptr[(size_t)(x + LAP) + (size_t)fl.pitch * ((size_t)(y + LAP) + (size_t)(k.ny + 2 * LAP) *
+                   ((size_t)(z + LAP) + (size_t)(k.nz + 2 * LAP) * m))] = r;
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy,
+                                             uint32_t sz)
+{
+    const long total = (long)sx * sy * sz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t x = (uint32_t)(i % sx), y = (uint32_t)((i / sx) % sy), z = (uint32_t)(i / ((long)sx * sy));
+    ptr[(size_t)x + (size_t)pitch * ((size_t)y + (size_t)z * sy)] = value;
+}
+
+// CFL: per-block maxima of max(|u|,|v|,|w|) + c over the interior, then one block
+// reduces them and writes dt.  No atomics, so the result is reproducible.
+__global__ void __launch_bounds__(NT) k_cfl_partial(KParams k, ws_field u, ws_field v, ws_field w, ws_field T,
+                                                    double *partial)
+{
+    __shared__ double red[NT];
+    const long total = (long)k.nx * k.ny * k.nz;
+    double mx = 0.0;
+    for (long i = (long)blockIdx.x * NT + threadIdx.x; i < total; i += (long)gridDim.x * NT) {
+        const int x = (int)(i % k.nx) + LAP;
+        const int y = (int)((i / k.nx) % k.ny) + LAP;
+        const int z = (int)(i / ((long)k.nx * k.ny)) + LAP;
+        const double c = sqrt(T.ptr[fidx(T.pitch, k.nyg, x, y, z)]) / k.ama;
+        const double a = fabs(u.ptr[fidx(u.pitch, k.nyg, x, y, z)]) + c;
+        const double b = fabs(v.ptr[fidx(v.pitch, k.nyg, x, y, z)]) + c;
+        const double e = fabs(w.ptr[fidx(w.pitch, k.nyg, x, y, z)]) + c;
+        mx = fmax(mx, fmax(a, fmax(b, e)));
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(NT) k_cfl_final(const double *partial, int n, double cfl, double hmin,
+                                                  double *dt_out)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[NT];
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) mx = fmax(mx, partial[i]);
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dt_out = cfl * hmin / red[0];
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
+
+}  // namespace
+
+int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s)
+{
+    const KParams &k = c.kp;
+    const long per = 2L * LAP * ((long)k.ny * k.nz + (long)k.nx * k.nz + (long)k.nx * k.ny);
+    for (int base = 0; base < n; base += 8) {
+        FieldList fl{};
+        const int cnt = n - base < 8 ? n - base : 8;
+        for (int i = 0; i < cnt; ++i) fl.f[i] = f[base + i];
+        hipLaunchKernelGGL(k_ghost_fill, dim3(blocks_for(per), cnt), dim3(NT), 0, s, fl, k.nx, k.ny, k.nz, k.nyg);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "ghost fill launch");
+}
+
+int launch_sound_speed(const Ctx &c, ws_field T, ws_field cc, ws_job job, hipStream_t s)
+{
+    const int bx = (int)(job.end.x - job.start.x), by = (int)(job.end.y - job.start.y),
+              bz = (int)(job.end.z - job.start.z);
+    hipLaunchKernelGGL(k_sound_speed, dim3(blocks_for((long)bx * by * bz)), dim3(NT), 0, s, T, cc, (int)job.start.x,
+                       (int)job.start.y, (int)job.start.z, bx, by, bz, c.kp.nyg, c.kp.ama);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "sound speed launch");
+}
+
+int launch_cons_to_pri(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T,
+                       ws_field P, hipStream_t s)
+{
+    const KParams &k = c.kp;
+    hipLaunchKernelGGL(k_cons_to_pri, dim3(blocks_for((long)k.nx * k.ny * k.nz)), dim3(NT), 0, s, k, f, d, u, v, w,
+                       T, P);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "cons_to_pri launch");
+}
+
+int launch_pri_to_cons(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T,
+                       ws_job job, hipStream_t s)
+{
+    const int bx = (int)(job.end.x - job.start.x), by = (int)(job.end.y - job.start.y),
+              bz = (int)(job.end.z - job.start.z);
+    hipLaunchKernelGGL(k_pri_to_cons, dim3(blocks_for((long)bx * by * bz)), dim3(NT), 0, s, c.kp, f, d, u, v, w, T,
+                       (int)job.start.x, (int)job.start.y, (int)job.start.z, bx, by, bz);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "pri_to_cons launch");
+}
+
+int launch_time_advance(const Ctx &c, int krk, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl, double dt,
+                        const double *dt_dev, hipStream_t s)
+{
+    const KParams &k = c.kp;
+    const dim3 g(blocks_for((long)k.nx * k.ny * k.nz));
+    if (krk == 1) hipLaunchKernelGGL(k_time_advance<1>, g, dim3(NT), 0, s, k, f, fn, du, fl, dt, dt_dev);
+    else if (krk == 2) hipLaunchKernelGGL(k_time_advance<2>, g, dim3(NT), 0, s, k, f, fn, du, fl, dt, dt_dev);
+    else hipLaunchKernelGGL(k_time_advance<3>, g, dim3(NT), 0, s, k, f, fn, du, fl, dt, dt_dev);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "time advance launch");
+}
+
+int launch_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy, uint32_t sz, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for((long)sx * sy * sz)), dim3(NT), 0, s, value, ptr, pitch, sx, sy, sz);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "fill launch");
+}
+
+int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream_t s)
+{
+    const KParams &k = c.kp;
+    if (!c.cfl_partial) {
+        c.cfl_blocks = 1024;
+        hipError_t e = hipMalloc(&c.cfl_partial, sizeof(double) * c.cfl_blocks);
+        if (e != hipSuccess) return hip_error(e, "cfl scratch allocation");
+    }
+    const double hmin = fmin(k.h[0], fmin(k.h[1], k.h[2]));
+    hipLaunchKernelGGL(k_cfl_partial, dim3(c.cfl_blocks), dim3(NT), 0, s, k, st.u, st.v, st.w, st.T, c.cfl_partial);
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(NT), 0, s, (const double *)c.cfl_partial, c.cfl_blocks, cfl, hmin,
+                       dt_dev);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "cfl launch");
+}
+
+}  // namespace ws

@@ -1,0 +1,226 @@
+// ws_device.h -- device-side building blocks of the inviscid RHS (FP64).
+//
+// Two forms of each scalar kernel live here:
+//   *_ref   the reference's operation order with contraction disabled, so that a
+//           kernel built from them reproduces the CPU restatement bit for bit;
+//   *_fast  the form the tiled kernels use: WENO smoothness written through
+//           forward differences, product weights through prefix/suffix
+//           products, and the division deferred so that the positive and
+//           negative reconstructions of a face share one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ws_internal.h"
+
+namespace ws {
+
+__device__ __forceinline__ size_t fidx(uint32_t pitch, int nyg, int x, int y, int z)
+{
+    return (size_t)x + (size_t)pitch * ((size_t)y + (size_t)nyg * (size_t)z);
+}
+
+__device__ __forceinline__ size_t sidx(uint32_t pitch, int ny, int nz, int x, int y, int z, int m)
+{
+    return (size_t)x + (size_t)pitch * ((size_t)y + (size_t)ny * ((size_t)z + (size_t)nz * (size_t)m));
+}
+
+// ---------------------------------------------------------------------------
+// Reference-order forms (bitwise anchor)
+// ---------------------------------------------------------------------------
+
+// WENO7-SYMBO positive reconstruction, OCFD_Schemes.cpp:306-368; s[0..7] = f(i-4..i+3),
+// result at face i-1/2.  `R` is an index map so that the negative rule reuses the text:
+// the negative reconstruction (OCFD_Schemes.cpp:491-553) is the positive rule on the
+// reversed window, which is exactly how the reference spells it index by index.
+template <bool REV>
+__device__ __forceinline__ double weno_ref(const double *w)
+{
+#pragma clang fp contract(off)
+    auto s = [&](int k) -> double { return REV ? w[7 - k] : w[k]; };
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0, S3 = 0.0, S4 = 0.0, t, t1;
+    t =  -2.0*s(0) +  9.0*s(1) - 18.0*s(2) + 11.0*s(3); S0 += 720.0*t*t;
+    t =       s(1) -  6.0*s(2) +  3.0*s(3) +  2.0*s(4); S1 += 720.0*t*t;
+    t =  -2.0*s(2) -  3.0*s(3) +  6.0*s(4) -      s(5); S2 += 720.0*t*t;
+    t = -11.0*s(3) + 18.0*s(4) -  9.0*s(5) +  2.0*s(6); S3 += 720.0*t*t;
+    t = -26.0*s(4) + 57.0*s(5) - 42.0*s(6) + 11.0*s(7); S4 += 720.0*t*t;
+
+    t = -6.0*s(0) + 24.0*s(1) - 30.0*s(2) + 12.0*s(3); S0 += 780.0*t*t;
+    t =                          6.0*s(2) - 12.0*s(3) +  6.0*s(4); S1 += 780.0*t*t;
+    t =                          6.0*s(3) - 12.0*s(4) +  6.0*s(5); S2 += 780.0*t*t;
+    t = 12.0*s(3) - 30.0*s(4) + 24.0*s(5) -  6.0*s(6); S3 += 780.0*t*t;
+    t = 18.0*s(4) - 48.0*s(5) + 42.0*s(6) - 12.0*s(7); S4 += 780.0*t*t;
+
+    t = -6.0*s(0) + 18.0*(s(1) - s(2)) + 6.0*s(3); S0 += 781.0*t*t;
+    t = -6.0*s(1) + 18.0*(s(2) - s(3)) + 6.0*s(4); S1 += 781.0*t*t;
+    t = -6.0*s(2) + 18.0*(s(3) - s(4)) + 6.0*s(5); S2 += 781.0*t*t;
+    t = -6.0*s(3) + 18.0*(s(4) - s(5)) + 6.0*s(6); S3 += 781.0*t*t;
+    t = -6.0*s(4) + 18.0*(s(5) - s(6)) + 6.0*s(7); S4 += 781.0*t*t;
+
+    t  = fmax(S0, S1);
+    t1 = fmax(S2, S3);
+    t  = fmax(t, t1);
+    S4 = fmax(t, S4);
+    {
+        double t2, t3;
+        t  = (0.0401954833730)*(2.592e-6+S1)*(2.592e-6+S2)*(2.592e-6+S3)*(2.592e-6+S4);
+        t1 = (0.2493800006710)*(2.592e-6+S0)*(2.592e-6+S2)*(2.592e-6+S3)*(2.592e-6+S4);
+        t2 = (0.4802686256260)*(2.592e-6+S0)*(2.592e-6+S1)*(2.592e-6+S3)*(2.592e-6+S4);
+        t3 = (0.2009775476730)*(2.592e-6+S0)*(2.592e-6+S1)*(2.592e-6+S2)*(2.592e-6+S4);
+        S4 = (0.0291783426580)*(2.592e-6+S0)*(2.592e-6+S1)*(2.592e-6+S2)*(2.592e-6+S3);
+        S3 = t3; S2 = t2; S1 = t1; S0 = t;
+    }
+    t1 = 0.0;
+    t = -3.0*s(0) + 13.0*s(1) - 23.0*s(2) + 25.0*s(3); t1 += S0*t;
+    t =      s(1) -  5.0*s(2) + 13.0*s(3) +  3.0*s(4); t1 += S1*t;
+    t =     -s(2) +  7.0*s(3) +  7.0*s(4) -      s(5); t1 += S2*t;
+    t =  3.0*s(3) + 13.0*s(4) -  5.0*s(5) +      s(6); t1 += S3*t;
+    t = 25.0*s(4) - 23.0*s(5) + 13.0*s(6) -  3.0*s(7); t1 += S4*t;
+    t1 /= (12.0*(S0+S1+S2+S3+S4));
+    return t1;
+}
+
+// Steger-Warming split, canonical form (OCFD_NS_Jacobian3d.cpp:69-127), no contraction.
+__device__ __forceinline__ void split_ref(const KParams &k, double d, double u, double v, double w,
+                                          double cc, double A1, double A2, double A3,
+                                          double fp[5], double fm[5])
+{
+#pragma clang fp contract(off)
+    double ss = sqrt(A1*A1 + A2*A2 + A3*A3);
+    double Ak1 = A1 / ss, Ak2 = A2 / ss, Ak3 = A3 / ss;
+    double vs = A1*u + A2*v + A3*w;
+    double E1 = vs, E2 = vs - cc*ss, E3 = vs + cc*ss;
+    double eps2 = k.eps2;
+    double E1P = (E1 + sqrt(E1*E1 + eps2)) * 0.50;
+    double E2P = (E2 + sqrt(E2*E2 + eps2)) * 0.50;
+    double E3P = (E3 + sqrt(E3*E3 + eps2)) * 0.50;
+    double E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+    double tmp0 = d / (2.0*k.gamma);
+    double uc1 = u - cc*Ak1, uc2 = u + cc*Ak1;
+    double vc1 = v - cc*Ak2, vc2 = v + cc*Ak2;
+    double wc1 = w - cc*Ak3, wc2 = w + cc*Ak3;
+    double vvc1 = (uc1*uc1 + vc1*vc1 + wc1*wc1) * 0.50;
+    double vvc2 = (uc2*uc2 + vc2*vc2 + wc2*wc2) * 0.50;
+    double vv = (k.gamma - 1.0) * (u*u + v*v + w*w);
+    double W2 = k.c3*cc*cc;
+    const double C1 = k.c1;
+    fp[0] = tmp0 * (C1*E1P + E2P + E3P);
+    fp[1] = tmp0 * (C1*E1P*u + E2P*uc1 + E3P*uc2);
+    fp[2] = tmp0 * (C1*E1P*v + E2P*vc1 + E3P*vc2);
+    fp[3] = tmp0 * (C1*E1P*w + E2P*wc1 + E3P*wc2);
+    fp[4] = tmp0 * (E1P*vv + E2P*vvc1 + E3P*vvc2 + W2*(E2P + E3P));
+    fm[0] = tmp0 * (C1*E1M + E2M + E3M);
+    fm[1] = tmp0 * (C1*E1M*u + E2M*uc1 + E3M*uc2);
+    fm[2] = tmp0 * (C1*E1M*v + E2M*vc1 + E3M*vc2);
+    fm[3] = tmp0 * (C1*E1M*w + E2M*wc1 + E3M*wc2);
+    fm[4] = tmp0 * (E1M*vv + E2M*vvc1 + E3M*vvc2 + W2*(E2M + E3M));
+}
+
+// ---------------------------------------------------------------------------
+// Fast forms (tiled kernels)
+// ---------------------------------------------------------------------------
+
+// Nonlinear weights are homogeneous of degree four in (eps + S_k), so all
+// indicators are scaled by 1/720 (eps 2.592e-6/720 = 3.6e-9) and the
+// second/third-derivative coefficients become 780*36/720 = 39 and 781*36/720 = 39.05.
+constexpr double kEpsW = 3.6e-9;
+constexpr double kB = 39.0;
+constexpr double kC = 39.05;
+constexpr double kD0 = 0.0401954833730;
+constexpr double kD1 = 0.2493800006710;
+constexpr double kD2 = 0.4802686256260;
+constexpr double kD3 = 0.2009775476730;
+constexpr double kD4 = 0.0291783426580;
+
+// Positive reconstruction at face i-1/2 from s[0..7] = f(i-4..i+3) (REV = false), or the
+// negative one (REV = true, the positive rule on the reversed window).  Returns the
+// unnormalised pair (num, den) with value = num / (12*den).
+template <bool REV, typename Load>
+__device__ __forceinline__ void weno_fast(Load ld, double &num, double &den)
+{
+    double s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = REV ? ld(7 - k) : ld(k);
+    double F1[7], F2[6], F3[5];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) F1[k] = s[k + 1] - s[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) F2[k] = F1[k + 1] - F1[k];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) F3[k] = F2[k + 1] - F2[k];
+    // first-derivative terms a_k = 6 F1_k + beta_k F2_k + gamma_k F3_k (anchored at s_k)
+    const double a0 = fma(11.0, F3[0], fma(15.0, F2[0], 6.0 * F1[0]));
+    const double a1 = fma(2.0, F3[1], fma(9.0, F2[1], 6.0 * F1[1]));
+    const double a2 = fma(-1.0, F3[2], fma(3.0, F2[2], 6.0 * F1[2]));
+    const double a3 = fma(2.0, F3[3], fma(-3.0, F2[3], 6.0 * F1[3]));
+    const double a4 = fma(11.0, F3[4], fma(-9.0, F2[4], 6.0 * F1[4]));
+    // second-derivative terms (divided by 6)
+    const double b0 = fma(2.0, F3[0], F2[0]);
+    const double b1 = F2[2];
+    const double b2 = F2[3];
+    const double b3 = F2[3] - F3[3];
+    const double b4 = fma(-2.0, F3[4], F2[4]);
+    double S0 = fma(a0, a0, fma(kB * b0, b0, (kC * F3[0]) * F3[0]));
+    double S1 = fma(a1, a1, fma(kB * b1, b1, (kC * F3[1]) * F3[1]));
+    double S2 = fma(a2, a2, fma(kB * b2, b2, (kC * F3[2]) * F3[2]));
+    double S3 = fma(a3, a3, fma(kB * b3, b3, (kC * F3[3]) * F3[3]));
+    double S4 = fma(a4, a4, fma(kB * b4, b4, (kC * F3[4]) * F3[4]));
+    S4 = fmax(fmax(fmax(S0, S1), fmax(S2, S3)), S4);
+    const double e0 = kEpsW + S0, e1 = kEpsW + S1, e2 = kEpsW + S2, e3 = kEpsW + S3, e4 = kEpsW + S4;
+    const double p01 = e0 * e1, p012 = p01 * e2, s34 = e3 * e4, s234 = e2 * s34;
+    const double w0 = kD0 * (e1 * s234);
+    const double w1 = kD1 * (e0 * s234);
+    const double w2 = kD2 * (p01 * s34);
+    const double w3 = kD3 * (p012 * e4);
+    const double w4 = kD4 * (p012 * e3);
+    const double q0 = fma(25.0, s[3], fma(-23.0, s[2], fma(13.0, s[1], -3.0 * s[0])));
+    const double q1 = fma(3.0, s[4], fma(13.0, s[3], fma(-5.0, s[2], s[1])));
+    const double q2 = fma(7.0, s[3] + s[4], -(s[2] + s[5]));
+    const double q3 = fma(13.0, s[4], fma(-5.0, s[5], fma(3.0, s[3], s[6])));
+    const double q4 = fma(25.0, s[4], fma(-23.0, s[5], fma(13.0, s[6], -3.0 * s[7])));
+    num = fma(w4, q4, fma(w3, q3, fma(w2, q2, fma(w1, q1, w0 * q0))));
+    den = ((w0 + w1) + (w2 + w3)) + w4;
+}
+
+// Steger-Warming split in the form the tiled kernels use: A1..A3 the metric terms,
+// sn = |A|, k1..k3 = A/|A| supplied by the caller (constants when metrics are uniform).
+__device__ __forceinline__ void split_fast(const KParams &k, double d, double u, double v, double w,
+                                           double cc, double A1, double A2, double A3, double sn,
+                                           double k1, double k2, double k3, double fp[5], double fm[5])
+{
+    const double vs = fma(A3, w, fma(A2, v, A1 * u));
+    const double cs = cc * sn;
+    const double E1 = vs, E2 = vs - cs, E3 = vs + cs;
+    const double E1P = 0.5 * (E1 + sqrt(fma(E1, E1, k.eps2)));
+    const double E2P = 0.5 * (E2 + sqrt(fma(E2, E2, k.eps2)));
+    const double E3P = 0.5 * (E3 + sqrt(fma(E3, E3, k.eps2)));
+    const double E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
+    const double t0 = d * k.half_rcp_gamma;
+    const double ck1 = cc * k1, ck2 = cc * k2, ck3 = cc * k3;
+    const double uc1 = u - ck1, uc2 = u + ck1;
+    const double vc1 = v - ck2, vc2 = v + ck2;
+    const double wc1 = w - ck3, wc2 = w + ck3;
+    const double h1 = 0.5 * fma(wc1, wc1, fma(vc1, vc1, uc1 * uc1));
+    const double h2 = 0.5 * fma(wc2, wc2, fma(vc2, vc2, uc2 * uc2));
+    const double vv = k.gm1 * fma(w, w, fma(v, v, u * u));
+    const double W2 = k.c3 * cc * cc;
+    // positive part
+    {
+        const double a = t0 * (k.c1 * E1P), b = t0 * E2P, c = t0 * E3P;
+        fp[0] = a + b + c;
+        fp[1] = fma(c, uc2, fma(b, uc1, a * u));
+        fp[2] = fma(c, vc2, fma(b, vc1, a * v));
+        fp[3] = fma(c, wc2, fma(b, wc1, a * w));
+        fp[4] = t0 * fma(W2, E2P + E3P, fma(E3P, h2, fma(E2P, h1, E1P * vv)));
+    }
+    {
+        const double a = t0 * (k.c1 * E1M), b = t0 * E2M, c = t0 * E3M;
+        fm[0] = a + b + c;
+        fm[1] = fma(c, uc2, fma(b, uc1, a * u));
+        fm[2] = fma(c, vc2, fma(b, vc1, a * v));
+        fm[3] = fma(c, wc2, fma(b, wc1, a * w));
+        fm[4] = t0 * fma(W2, E2M + E3M, fma(E3M, h2, fma(E2M, h1, E1M * vv)));
+    }
+}
+
+}  // namespace ws

@@ -1,0 +1,73 @@
+// ws_internal.h -- internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/windstencil_amd.h"
+
+namespace ws {
+
+constexpr int LAP = WS_LAP;
+
+// Everything a kernel needs about the grid and the physics, passed by value.
+struct KParams {
+    int nx, ny, nz;            // interior extents
+    int nyg, nzg;              // ghosted extents in y and z (n + 2*LAP)
+    double gamma;
+    double c1;                 // 2(gamma-1)
+    double c3;                 // (3-gamma)/(2(gamma-1))
+    double half_rcp_gamma;     // 0.5/gamma
+    double gm1;                // gamma-1
+    double eps2;               // epsl_sw^2
+    double ama, rcp_ama;
+    double cv;                 // 1/(gamma (gamma-1) Ma^2)
+    double h[3];               // grid spacings
+    double rh[3];              // 1/h
+    // constant metrics: direction d uses A[3d..3d+2]; their norm and unit vector
+    double A[9];
+    double Anorm[3];
+    double Ak[9];
+    double ajac;
+};
+
+struct Ctx {
+    ws_config cfg;
+    KParams kp;
+    int device;
+    // CFL scratch: per-block maxima
+    double *cfl_partial = nullptr;
+    int cfl_blocks = 0;
+    // private stream for graph capture (the caller's stream may be the legacy
+    // default stream, which cannot be captured); default priority
+    hipStream_t capture_stream = nullptr;
+};
+
+KParams make_kparams(const ws_config &cfg);
+
+// error reporting (thread-local message)
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *what);
+
+// ---- launchers (defined next to their kernels) ----
+int launch_rhs_anchor(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
+int launch_rhs_tiled(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                     const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
+int launch_rhs_mixed(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                     const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
+
+int launch_sound_speed(const Ctx &c, ws_field T, ws_field cc, ws_job job, hipStream_t s);
+int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s);
+int launch_cons_to_pri(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w,
+                       ws_field T, ws_field P, hipStream_t s);
+int launch_pri_to_cons(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w,
+                       ws_field T, ws_job job, hipStream_t s);
+int launch_time_advance(const Ctx &c, int krk, ws_soa f, ws_soa fn, ws_soa du, ws_soa f_lap,
+                        double dt, const double *dt_dev, hipStream_t s);
+int launch_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy, uint32_t sz,
+                hipStream_t s);
+int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream_t s);
+
+}  // namespace ws
