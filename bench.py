@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline leg")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--tile", type=int, default=-1, help="force a tile shape (default: autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for HBM counter calibration)")
     return ap.parse_args()
@@ -135,6 +136,17 @@ def main():
         if ev is not None:
             ev[2].record(stream)
 
+    rhs()
+    torch.cuda.synchronize()
+    tune = None
+    if args.tile >= 0:
+        W.check(L.ws_ctx_set_tile_shape(ctx.handle, args.tile))
+    else:
+        times = (ctypes.c_float * len(W.TILE_SHAPES))()
+        W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 5, times, sh))
+        tune = {str(s): round(times[s], 4) for s in W.TILE_SHAPES}
+    chosen = ctypes.c_int32()
+    W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
     for _ in range(args.warmup):
         rhs()
     torch.cuda.synchronize()
@@ -196,6 +208,7 @@ def main():
                      "kernel": "k_rhs_tile x3 directions (one RHS)",
                      "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
         "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
+        "tile_shape": chosen.value, "autotune_ms": tune,
         "parity_rel_l2_vs_anchor": parity,
     }
     if not args.no_cpu:

@@ -48,6 +48,11 @@ extern "C" {
 #define WS_KERNEL_ANCHOR 1       /* per-direction split + per-cell WENO kernels, reference operation
                                     order, no contraction: bitwise equal to the CPU restatement */
 
+/* Tile shapes of the tiled FP64 kernels (cells per line x lines per block, faces per thread) */
+#define WS_TILE_15x16_K1 0       /* x: 15x16, y/z: 16x15, one face per thread */
+#define WS_TILE_30x16_K2 1       /* x: 30x16, y/z: 32x15, two faces per thread (default) */
+#define WS_TILE_COUNT 2
+
 /* Ghosted scalar field; layout of cudaField under __HIPCC__ (cuda_commen.h:48-54). */
 typedef struct ws_field {
     double *ptr;
@@ -120,6 +125,8 @@ int ws_ctx_destroy(ws_ctx *ctx);
 int ws_ctx_get_config(const ws_ctx *ctx, ws_config *out);
 int ws_ctx_set_precision(ws_ctx *ctx, int32_t precision);
 int ws_ctx_set_kernel(ws_ctx *ctx, int32_t kernel);
+int ws_ctx_set_tile_shape(ws_ctx *ctx, int32_t shape);
+int ws_ctx_get_tile_shape(const ws_ctx *ctx, int32_t *shape);
 
 /* Allocate / free the solver state with the reference's extents
  * (replaces opencfd_mem_init_dev, src/runtime/OCFD_init.cpp:411-446). */
@@ -184,6 +191,12 @@ int ws_mem_value_init(double value, double *ptr, uint32_t pitch, uint32_t size_x
 /* CFL step dt = cfl*min(h)/max_cells max_dir(|u_dir| + c) written to *dt_dev (a device
  * double).  No counterpart: the reference uses a fixed DT. */
 int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipStream_t stream);
+
+/* Time every tile shape of the tiled FP64 kernels on this state (two untimed and `reps`
+ * timed RHS launches each, HIP events on `stream`), keep the fastest for the context and
+ * remember it for contexts of the same extents.  times_ms (WS_TILE_COUNT entries) may be
+ * NULL.  Overwrites st->du; synchronises the stream.  No counterpart in the reference. */
+int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
 
 /* ---- time integration ---------------------------------------------------- */
 

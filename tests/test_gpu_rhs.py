@@ -33,18 +33,50 @@ def test_anchor_is_bitwise_equal_to_oracle(orc, ws, shape, kind):
     assert np.array_equal(du, ref), f"max |diff| {np.abs(du - ref).max()} rel L2 {rel_l2(du, ref)}"
 
 
-@pytest.mark.parametrize("shape", SMALL)
+@pytest.mark.parametrize("tile", W.TILE_SHAPES)
+@pytest.mark.parametrize("shape", SMALL + [(67, 45, 37)])
 @pytest.mark.parametrize("kind", KINDS)
-def test_tiled_fp64_matches_oracle(orc, ws, shape, kind):
+def test_tiled_fp64_matches_oracle(orc, ws, shape, kind, tile):
     nx, ny, nz = shape
-    log(f"tiled {kind} {shape}")
+    log(f"tiled {kind} {shape} tile {tile}")
     prims = random_state(nx, ny, nz, 23, kind)
     ctx = make_ctx(nx, ny, nz)
+    ctx.set_tile_shape(tile)
     du = gpu_du_comput(ctx, upload_state(prims))
     ref = oracle_du(orc, prims)
     err = rel_l2(du, ref)
     log(f"  rel L2 {err:.3e}")
     assert err <= TOL_FP64
+
+
+def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
+    """All tile shapes give the same RHS to rounding; the autotuned context still
+    matches the oracle and remembers its choice for the grid."""
+    import ctypes
+    nx, ny, nz = 90, 64, 32
+    prims = random_state(nx, ny, nz, 31, "shock")
+    ref = oracle_du(orc, prims)
+    outs = []
+    for tile in W.TILE_SHAPES:
+        ctx = make_ctx(nx, ny, nz)
+        ctx.set_tile_shape(tile)
+        outs.append(gpu_du_comput(ctx, upload_state(prims)))
+    for o in outs[1:]:
+        assert rel_l2(o, outs[0]) <= 1e-13
+    ctx = make_ctx(nx, ny, nz)
+    st = upload_state(prims)
+    gpu_du_comput(ctx, st)
+    times = (ctypes.c_float * len(W.TILE_SHAPES))()
+    W.check(W.lib().ws_autotune(ctx.handle, ctypes.byref(st.ws), 3, times, W.stream_handle()))
+    log(f"  autotune ms {list(times)}")
+    chosen = ctypes.c_int32()
+    W.check(W.lib().ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
+    assert times[chosen.value] == min(times)
+    ctx2 = make_ctx(nx, ny, nz)
+    again = ctypes.c_int32()
+    W.check(W.lib().ws_ctx_get_tile_shape(ctx2.handle, ctypes.byref(again)))
+    assert again.value == chosen.value
+    assert rel_l2(gpu_du_comput(ctx2, upload_state(prims)), ref) <= TOL_FP64
 
 
 def test_tiled_is_independent_of_pitch(ws):

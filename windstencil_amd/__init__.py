@@ -15,6 +15,8 @@ LAP = 4
 WS_OK = 0
 WS_PREC_FP64, WS_PREC_MIXED = 0, 1
 WS_KERNEL_TILED, WS_KERNEL_ANCHOR = 0, 1
+WS_TILE_15x16_K1, WS_TILE_30x16_K2 = 0, 1
+TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2)
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32
@@ -67,6 +69,8 @@ _SIGS = {
     "ws_ctx_get_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsConfig)]),
     "ws_ctx_set_precision": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_ctx_set_kernel": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_ctx_set_tile_shape": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_ctx_get_tile_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_state_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_state_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p] + [WsField] * 5 +
@@ -84,6 +88,8 @@ _SIGS = {
                                          ctypes.c_void_p]),
     "ws_cfl_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_double, ctypes.c_void_p,
                                  ctypes.c_void_p]),
+    "ws_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.POINTER(ctypes.c_float),
+                                   ctypes.c_void_p]),
     "ws_rk3_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_void_p]),
     "ws_rk3_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.c_double, _i32,
                                   ctypes.c_void_p]),
@@ -177,6 +183,9 @@ class Context:
 
     def set_kernel(self, k):
         check(lib().ws_ctx_set_kernel(self.handle, k), "ws_ctx_set_kernel")
+
+    def set_tile_shape(self, s):
+        check(lib().ws_ctx_set_tile_shape(self.handle, s), "ws_ctx_set_tile_shape")
 
 
 # ---- torch-backed buffers ---------------------------------------------------------------------

@@ -1,0 +1,75 @@
+// autotune.cpp -- pick the tile shape of the tiled FP64 kernels by timing them.
+//
+// Every candidate shape runs the full inviscid RHS on the caller's state: two
+// untimed launches, then `reps` launches bracketed by HIP events on the caller's
+// stream.  The fastest shape becomes the context's shape and is remembered for the
+// grid extents, so later contexts of the same size start from it.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "ws_internal.h"
+
+using namespace ws;
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int>, int> g_best;   // (nx, ny, nz) -> shape
+
+}  // namespace
+
+namespace ws {
+
+int remembered_shape(int nx, int ny, int nz)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_best.find(std::make_tuple(nx, ny, nz));
+    return it == g_best.end() ? -1 : it->second;
+}
+
+}  // namespace ws
+
+extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *times_ms, hipStream_t s)
+{
+    if (!c || !st || reps <= 0) return set_error(WS_ERR_INVALID, "bad autotune request");
+    ws_job job;
+    job.start = {(uint32_t)LAP, (uint32_t)LAP, (uint32_t)LAP};
+    job.end = {(uint32_t)(c->cfg.nx + LAP), (uint32_t)(c->cfg.ny + LAP), (uint32_t)(c->cfg.nz + LAP)};
+    hipEvent_t e0, e1;
+    hipError_t e = hipEventCreate(&e0);
+    if (e != hipSuccess) return hip_error(e, "autotune event");
+    e = hipEventCreate(&e1);
+    if (e != hipSuccess) { hipEventDestroy(e0); return hip_error(e, "autotune event"); }
+    const int saved = c->tile_shape;
+    int best = saved, r = WS_OK;
+    float best_ms = 1e30f;
+    for (int shape = 0; shape < WS_TILE_COUNT && !r; ++shape) {
+        c->tile_shape = shape;
+        for (int i = 0; i < 2 && !r; ++i)
+            r = launch_rhs_tiled(*c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, job, 0, s);
+        if (r) break;
+        hipEventRecord(e0, s);
+        for (int i = 0; i < reps && !r; ++i)
+            r = launch_rhs_tiled(*c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, job, 0, s);
+        hipEventRecord(e1, s);
+        if (r) break;
+        e = hipEventSynchronize(e1);
+        if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (times_ms) times_ms[shape] = ms;
+        if (ms < best_ms) { best_ms = ms; best = shape; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->tile_shape = r ? saved : best;
+    if (!r) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_best[std::make_tuple(c->cfg.nx, c->cfg.ny, c->cfg.nz)] = best;
+    }
+    return r;
+}

@@ -9,8 +9,6 @@
 
 #include "ws_internal.h"
 
-struct ws_ctx : public ws::Ctx {};
-
 namespace ws {
 
 static thread_local std::string g_last_error;
@@ -223,6 +221,8 @@ int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
     if (!c) return set_error(WS_ERR_NOMEM, "host allocation failed");
     c->cfg = *cfg;
     c->kp = make_kparams(*cfg);
+    const int shape = remembered_shape(cfg->nx, cfg->ny, cfg->nz);
+    if (shape >= 0) c->tile_shape = shape;
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return hip_error(e, "hipGetDevice"); }
     c->cfl_blocks = 1024;
@@ -263,6 +263,21 @@ int ws_ctx_set_kernel(ws_ctx *c, int32_t kernel)
     if (!c) return set_error(WS_ERR_INVALID, "null context");
     if (kernel != WS_KERNEL_TILED && kernel != WS_KERNEL_ANCHOR) return set_error(WS_ERR_INVALID, "unknown kernel");
     c->cfg.kernel = kernel;
+    return WS_OK;
+}
+
+int ws_ctx_set_tile_shape(ws_ctx *c, int32_t shape)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (shape < 0 || shape >= WS_TILE_COUNT) return set_error(WS_ERR_INVALID, "unknown tile shape");
+    c->tile_shape = shape;
+    return WS_OK;
+}
+
+int ws_ctx_get_tile_shape(const ws_ctx *c, int32_t *shape)
+{
+    if (!c || !shape) return set_error(WS_ERR_INVALID, "null argument");
+    *shape = c->tile_shape;
     return WS_OK;
 }
 

@@ -1,18 +1,19 @@
 // k_rhs_tiled.hip -- LDS-tiled inviscid RHS, FP64 (the fast path).
 //
-// One launch per direction.  A 256-thread block owns a tile of one plane:
-//   x-direction: 15 cells along x (16 faces) x 16 rows along y, at one z;
-//   y-direction: 15 columns along x x 16 cells along y (17 faces), at one z;
-//   z-direction: 15 columns along x x 16 cells along z (17 faces), at one y.
-// Phase 1 splits every point of the tile's lines, halo included, into LDS
-// (10 doubles per point).  Phase 2 gives each thread one face: it reconstructs
-// all five variables with the positive and the negative rule and merges each
-// pair over one division.  Phase 3 turns face fluxes into cell updates.
-// Tile widths divide the benchmark grids (600 = 40*15, 256 = 16*16, 128 = 8*16);
-// other sizes leave a partial last tile, masked.
+// One launch per direction.  A block owns a plane tile of W lines of L cells:
+//   x-direction: lines along x (rows), W rows along y, at one z;
+//   y-direction: lines along y, W columns along x, at one z;
+//   z-direction: lines along z, W columns along x, at one y.
+// Phase 1 splits every point of the tile's lines, halo included (L+8 points per
+// line), into LDS: 10 doubles per point.  Phase 2 gives each thread K consecutive
+// faces of one line; for every variable it reconstructs them with the positive and
+// the negative rule (sharing differences between the K faces) and merges each pair
+// over one reciprocal.  Phase 3 turns face fluxes into cell updates.
+// The shapes are template parameters; the library picks one per grid (autotune.cpp).
 #include <hip/hip_runtime.h>
 
 #include "ws_device.h"
+#include "ws_tiles.h"
 
 namespace ws {
 
@@ -23,172 +24,198 @@ struct Box {
     int e[3];   // end
 };
 
-constexpr int NT = 256;          // threads per block
-constexpr int TA = 15;           // cells along the line (x) / columns across (y, z)
-constexpr int TL = 16;           // rows (x) / cells along the line (y, z)
-constexpr int QS = 384;          // LDS entries per quantity (>= 23*16 and 24*16)
+template <int DIR, int L, int W, int K>
+struct Tile {
+    static constexpr int NL = L + 2 * LAP;                // points per line
+    static constexpr int F = L + 1;                        // faces per line
+    static constexpr int T = (F + K - 1) / K;              // threads per line
+    static constexpr int NT = W * T;                       // threads per block
+    static constexpr int PS = DIR == 0 ? NL : W + 1;       // LDS stride: x rows / padded columns
+    static constexpr int QS = DIR == 0 ? W * NL : NL * (W + 1);   // LDS entries per quantity
+    static constexpr int NPTS = W * NL;
+    // the last thread of a line may read up to K+6 points past its first face, i.e.
+    // past the line's end; the pad keeps those reads inside the buffer (their values
+    // only reach faces that are never stored)
+    static constexpr int PAD = (K + 1) * (W + 1) + 8;
+    static_assert(NT <= 256, "block too large");
+    static_assert(5 * W * F <= 10 * QS, "face buffer must fit in the split buffer");
+    // LDS index of point p (0..NL-1) of line l (0..W-1)
+    __device__ static int pidx(int l, int p) { return DIR == 0 ? l * NL + p : p * (W + 1) + l; }
+    // face buffer index of face f of line l
+    __device__ static int fidx(int l, int f) { return DIR == 0 ? l * F + f : f * W + l; }
+};
 
-template <int DIR, bool FIELDS, bool ACC>
-__global__ void __launch_bounds__(NT) k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w,
-                                                 ws_field cc, ws_metric_fields mf, ws_soa du, Box box)
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC>
+__global__ void __launch_bounds__(256)
+k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_metric_fields mf,
+           ws_soa du, Box box)
 {
-    __shared__ double sp[10 * QS];
+    using TL = Tile<DIR, L, W, K>;
+    __shared__ double sp[10 * TL::QS + TL::PAD];
     const int t = threadIdx.x;
 
-    // tile origin
+    // tile origin: x0 always tiles x; the line axis is x (DIR 0), y (DIR 1) or z (DIR 2)
     int x0, y0, z0;
     if (DIR == 0) {
-        x0 = box.s[0] + blockIdx.x * TA;
-        y0 = box.s[1] + blockIdx.y * TL;
+        x0 = box.s[0] + blockIdx.x * L;
+        y0 = box.s[1] + blockIdx.y * W;
         z0 = box.s[2] + blockIdx.z;
     } else if (DIR == 1) {
-        x0 = box.s[0] + blockIdx.x * TA;
-        y0 = box.s[1] + blockIdx.y * TL;
+        x0 = box.s[0] + blockIdx.x * W;
+        y0 = box.s[1] + blockIdx.y * L;
         z0 = box.s[2] + blockIdx.z;
     } else {
-        x0 = box.s[0] + blockIdx.x * TA;
-        z0 = box.s[2] + blockIdx.y * TL;
+        x0 = box.s[0] + blockIdx.x * W;
+        z0 = box.s[2] + blockIdx.y * L;
         y0 = box.s[1] + blockIdx.z;
     }
+    // cells of the tile that lie in the box, along the line and across lines
+    const int lim_line = DIR == 0 ? box.e[0] - x0 : DIR == 1 ? box.e[1] - y0 : box.e[2] - z0;
+    const int lim_across = DIR == 0 ? box.e[1] - y0 : box.e[0] - x0;
 
-    // constant-metric split coefficients of this direction
     const double cA1 = k.A[3 * DIR + 0], cA2 = k.A[3 * DIR + 1], cA3 = k.A[3 * DIR + 2];
     const double cSn = k.Anorm[DIR];
     const double cK1 = k.Ak[3 * DIR + 0], cK2 = k.Ak[3 * DIR + 1], cK3 = k.Ak[3 * DIR + 2];
 
-    // ---- phase 1: split every point of the tile ---------------------------------
-    constexpr int NPTS = DIR == 0 ? (TA + 2 * LAP) * TL : TA * (TL + 2 * LAP);
-    for (int pt = t; pt < NPTS; pt += NT) {
-        int x, y, z, li;
-        bool ok;
-        if (DIR == 0) {
-            const int px = pt % (TA + 2 * LAP), r = pt / (TA + 2 * LAP);
-            x = x0 - LAP + px; y = y0 + r; z = z0;
-            li = r * 24 + px;
-            ok = (y < box.e[1]) && (x < box.e[0] + LAP);
+    // ---- phase 1: split every point of the tile ----------------------------------------
+    for (int pt = t; pt < TL::NPTS; pt += TL::NT) {
+        int l, p;
+        if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }     // x fastest
+        else          { l = pt % W;      p = pt / W; }          // x fastest
+        // the line needs points -LAP .. lim_line+LAP-1 and lines 0 .. lim_across-1
+        if (l >= lim_across || p >= lim_line + 2 * LAP) continue;
+        int x, y, z;
+        if (DIR == 0)      { x = x0 - LAP + p; y = y0 + l; z = z0; }
+        else if (DIR == 1) { x = x0 + l; y = y0 - LAP + p; z = z0; }
+        else               { x = x0 + l; y = y0; z = z0 - LAP + p; }
+        const int nyg = k.nyg;
+        const double dd = d.ptr[fidx(d.pitch, nyg, x, y, z)];
+        const double uu = u.ptr[fidx(u.pitch, nyg, x, y, z)];
+        const double vv = v.ptr[fidx(v.pitch, nyg, x, y, z)];
+        const double ww = w.ptr[fidx(w.pitch, nyg, x, y, z)];
+        const double c = cc.ptr[fidx(cc.pitch, nyg, x, y, z)];
+        double fp[5], fm[5];
+        if (FIELDS) {
+            const ws_field &f1 = mf.A[3 * DIR + 0], &f2 = mf.A[3 * DIR + 1], &f3 = mf.A[3 * DIR + 2];
+            const double A1 = f1.ptr[fidx(f1.pitch, nyg, x, y, z)];
+            const double A2 = f2.ptr[fidx(f2.pitch, nyg, x, y, z)];
+            const double A3 = f3.ptr[fidx(f3.pitch, nyg, x, y, z)];
+            const double sn = sqrt(fma(A3, A3, fma(A2, A2, A1 * A1)));
+            const double r = 1.0 / sn;
+            split_fast(k, dd, uu, vv, ww, c, A1, A2, A3, sn, A1 * r, A2 * r, A3 * r, fp, fm);
         } else {
-            const int a = pt % TA, p = pt / TA;
-            x = x0 + a;
-            if (DIR == 1) { y = y0 - LAP + p; z = z0; ok = (y < box.e[1] + LAP); }
-            else          { z = z0 - LAP + p; y = y0; ok = (z < box.e[2] + LAP); }
-            ok = ok && (x < box.e[0]);
-            li = p * 16 + a;
+            split_fast(k, dd, uu, vv, ww, c, cA1, cA2, cA3, cSn, cK1, cK2, cK3, fp, fm);
         }
-        if (ok) {
-            const int nyg = k.nyg;
-            const double dd = d.ptr[fidx(d.pitch, nyg, x, y, z)];
-            const double uu = u.ptr[fidx(u.pitch, nyg, x, y, z)];
-            const double vv = v.ptr[fidx(v.pitch, nyg, x, y, z)];
-            const double ww = w.ptr[fidx(w.pitch, nyg, x, y, z)];
-            const double c = cc.ptr[fidx(cc.pitch, nyg, x, y, z)];
-            double fp[5], fm[5];
-            if (FIELDS) {
-                const ws_field &f1 = mf.A[3 * DIR + 0], &f2 = mf.A[3 * DIR + 1], &f3 = mf.A[3 * DIR + 2];
-                const double A1 = f1.ptr[fidx(f1.pitch, nyg, x, y, z)];
-                const double A2 = f2.ptr[fidx(f2.pitch, nyg, x, y, z)];
-                const double A3 = f3.ptr[fidx(f3.pitch, nyg, x, y, z)];
-                const double sn = sqrt(fma(A3, A3, fma(A2, A2, A1 * A1)));
-                const double r = 1.0 / sn;
-                split_fast(k, dd, uu, vv, ww, c, A1, A2, A3, sn, A1 * r, A2 * r, A3 * r, fp, fm);
-            } else {
-                split_fast(k, dd, uu, vv, ww, c, cA1, cA2, cA3, cSn, cK1, cK2, cK3, fp, fm);
-            }
+        const int li = TL::pidx(l, p);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                sp[m * QS + li] = fp[m];
-                sp[(5 + m) * QS + li] = fm[m];
-            }
+        for (int m = 0; m < 5; ++m) {
+            sp[m * TL::QS + li] = fp[m];
+            sp[(5 + m) * TL::QS + li] = fm[m];
         }
     }
     __syncthreads();
 
-    // ---- phase 2: one face per thread, all five variables --------------------------
-    double F[5];
-    bool face_ok;
-    int base, stride;
-    if (DIR == 0) {
-        const int f = t % 16, r = t / 16;
-        face_ok = (y0 + r < box.e[1]) && (x0 + f <= box.e[0]);
-        base = r * 24 + f;
-        stride = 1;
-    } else {
-        const int a = t % TA, f = t / TA;      // f in 0..16 for t < 255
-        const int lim = DIR == 1 ? box.e[1] - y0 : box.e[2] - z0;
-        face_ok = (t < TA * (TL + 1)) && (x0 + a < box.e[0]) && (f <= lim);
-        base = f * 16 + a;
-        stride = 16;
-    }
-    if (face_ok) {
+    // ---- phase 2: K faces of one line per thread, all five variables --------------------
+    int line, seg;
+    if (DIR == 0) { line = t / TL::T; seg = t % TL::T; }
+    else          { line = t % W;     seg = t / W; }
+    const int f0 = seg * K;                               // first face of this thread
+    const bool active = (t < TL::NT) && (line < lim_across) && (f0 <= lim_line);
+    double Fv[5][K];
+    if (active) {
+        const int base = TL::pidx(line, f0);
+        const int step = DIR == 0 ? 1 : W + 1;
 #pragma unroll 1
         for (int m = 0; m < 5; ++m) {
-            const double *qp = sp + m * QS + base;
-            const double *qm = sp + (5 + m) * QS + base;
-            double nP, dP, nM, dM;
-            weno_fast<false>([&](int j) { return qp[j * stride]; }, nP, dP);
-            weno_fast<true>([&](int j) { return qm[j * stride]; }, nM, dM);
-            F[m] = fma(nP, dM, nM * dP) / (12.0 * (dP * dM));
+            const double *qp = sp + m * TL::QS + base;
+            const double *qm = sp + (5 + m) * TL::QS + base;
+            // a stored face f (f <= lim_line) reads only points f..f+7, all written in
+            // phase 1; faces past lim_line read stale LDS and are not stored
+            double nP[K], dP[K], nM[K], dM[K];
+            weno_k<false, K>([&](int j) { return qp[j * step]; }, nP, dP);
+            weno_k<true, K>([&](int j) { return qm[j * step]; }, nM, dM);
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const double den = dP[j] * dM[j];
+                Fv[m][j] = fma(nP[j], dM[j], nM[j] * dP[j]) * recip_nr(den);
+            }
         }
     }
     __syncthreads();
-    // face fluxes reuse the split buffer: face[m][t]
-    if (face_ok) {
+    // face fluxes reuse the split buffer
+    if (active) {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) sp[m * NT + t] = F[m];
+        for (int j = 0; j < K; ++j) {
+            if (f0 + j > lim_line || f0 + j >= TL::F) break;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) sp[m * W * TL::F + TL::fidx(line, f0 + j)] = Fv[m][j];
+        }
     }
     __syncthreads();
 
-    // ---- phase 3: flux difference of each cell ---------------------------------------
-    int cx, cy, cz, fl, fr;
-    bool cell_ok;
-    if (DIR == 0) {
-        const int c = t % 16, r = t / 16;
-        cx = x0 + c; cy = y0 + r; cz = z0;
-        cell_ok = (c < TA) && (cx < box.e[0]) && (cy < box.e[1]);
-        fl = r * 16 + c;
-        fr = fl + 1;
-    } else {
-        const int a = t % TA, c = t / TA;      // c in 0..15 for t < 240
-        cx = x0 + a;
-        if (DIR == 1) { cy = y0 + c; cz = z0; cell_ok = (cy < box.e[1]); }
-        else          { cz = z0 + c; cy = y0; cell_ok = (cz < box.e[2]); }
-        cell_ok = cell_ok && (t < TA * TL) && (cx < box.e[0]);
-        fl = c * TA + a;
-        fr = fl + TA;
-    }
-    if (cell_ok) {
+    // ---- phase 3: flux difference of each cell ---------------------------------------------
+    const double scale0 = -k.rh[DIR];
+    const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
+    for (int c = t; c < W * L; c += TL::NT) {
+        int l, i;
+        if (DIR == 0) { i = c % L; l = c / L; }
+        else          { l = c % W; i = c / W; }
+        if (l >= lim_across || i >= lim_line) continue;
+        int cx, cy, cz;
+        if (DIR == 0)      { cx = x0 + i; cy = y0 + l; cz = z0; }
+        else if (DIR == 1) { cx = x0 + l; cy = y0 + i; cz = z0; }
+        else               { cx = x0 + l; cy = y0; cz = z0 + i; }
         const double aj = FIELDS ? mf.ajac.ptr[fidx(mf.ajac.pitch, k.nyg, cx, cy, cz)] : k.ajac;
-        const double scale = -aj * k.rh[DIR];
-        const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
-        double *dp = du.ptr + (size_t)(cx - LAP) + (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
+        const double scale = aj * scale0;
+        double *dp = du.ptr + (size_t)(cx - LAP) +
+                     (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
+        const int fl = TL::fidx(l, i), fr = TL::fidx(l, i + 1);
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-            const double val = scale * (sp[m * NT + fr] - sp[m * NT + fl]);
+            const double val = scale * (sp[m * W * TL::F + fr] - sp[m * W * TL::F + fl]);
             if (ACC) dp[m * vstride] += val;
             else dp[m * vstride] = val;
         }
     }
 }
 
-template <int DIR, bool FIELDS, bool ACC>
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC>
 void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                 const ws_metric_fields &mf, ws_soa du, const Box &box, hipStream_t s)
 {
+    using TL = Tile<DIR, L, W, K>;
     const int nx = box.e[0] - box.s[0], ny = box.e[1] - box.s[1], nz = box.e[2] - box.s[2];
     dim3 grid;
-    if (DIR == 0) grid = dim3((nx + TA - 1) / TA, (ny + TL - 1) / TL, nz);
-    else if (DIR == 1) grid = dim3((nx + TA - 1) / TA, (ny + TL - 1) / TL, nz);
-    else grid = dim3((nx + TA - 1) / TA, (nz + TL - 1) / TL, ny);
-    hipLaunchKernelGGL((k_rhs_tile<DIR, FIELDS, ACC>), grid, dim3(NT), 0, s, k, d, u, v, w, cc, mf, du, box);
+    if (DIR == 0) grid = dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
+    else if (DIR == 1) grid = dim3((nx + W - 1) / W, (ny + L - 1) / L, nz);
+    else grid = dim3((nx + W - 1) / W, (nz + L - 1) / L, ny);
+    hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC>), grid, dim3(TL::NT), 0, s, k, d, u, v, w, cc, mf,
+                       du, box);
 }
 
-template <bool FIELDS>
+template <int L0, int W0, int L1, int W1, int K, bool FIELDS>
 void launch_all(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                 const ws_metric_fields &mf, ws_soa du, const Box &box, int accumulate, hipStream_t s)
 {
-    if (accumulate) launch_dir<0, FIELDS, true>(k, d, u, v, w, cc, mf, du, box, s);
-    else launch_dir<0, FIELDS, false>(k, d, u, v, w, cc, mf, du, box, s);
-    launch_dir<1, FIELDS, true>(k, d, u, v, w, cc, mf, du, box, s);
-    launch_dir<2, FIELDS, true>(k, d, u, v, w, cc, mf, du, box, s);
+    if (accumulate) launch_dir<0, L0, W0, K, FIELDS, true>(k, d, u, v, w, cc, mf, du, box, s);
+    else launch_dir<0, L0, W0, K, FIELDS, false>(k, d, u, v, w, cc, mf, du, box, s);
+    launch_dir<1, L1, W1, K, FIELDS, true>(k, d, u, v, w, cc, mf, du, box, s);
+    launch_dir<2, L1, W1, K, FIELDS, true>(k, d, u, v, w, cc, mf, du, box, s);
+}
+
+template <bool FIELDS>
+void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                  const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, hipStream_t s)
+{
+    switch (shape) {
+    case WS_TILE_15x16_K1:  // x: 15 cells x 16 rows; y, z: 16 cells x 15 columns; one face per thread
+        launch_all<15, 16, 16, 15, 1, FIELDS>(k, d, u, v, w, cc, mf, du, box, acc, s);
+        break;
+    case WS_TILE_30x16_K2:  // x: 30 cells x 16 rows; y, z: 32 cells x 15 columns; two faces per thread
+    default:
+        launch_all<30, 16, 32, 15, 2, FIELDS>(k, d, u, v, w, cc, mf, du, box, acc, s);
+        break;
+    }
 }
 
 }  // namespace
@@ -200,11 +227,12 @@ int launch_rhs_tiled(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field 
     box.s[0] = (int)job.start.x; box.s[1] = (int)job.start.y; box.s[2] = (int)job.start.z;
     box.e[0] = (int)job.end.x; box.e[1] = (int)job.end.y; box.e[2] = (int)job.end.z;
     ws_metric_fields mf{};
+    const int shape = c.tile_shape;
     if (m) {
         mf = *m;
-        launch_all<true>(c.kp, d, u, v, w, cc, mf, du, box, accumulate, s);
+        launch_shape<true>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, s);
     } else {
-        launch_all<false>(c.kp, d, u, v, w, cc, mf, du, box, accumulate, s);
+        launch_shape<false>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "tiled RHS launch");

@@ -182,6 +182,102 @@ __device__ __forceinline__ void weno_fast(Load ld, double &num, double &den)
     den = ((w0 + w1) + (w2 + w3)) + w4;
 }
 
+// Reciprocal: hardware estimate plus two Newton steps (error within a couple of ulp;
+// the argument is a positive normal number wherever it is used).
+__device__ __forceinline__ double recip_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Square root of a positive normal number: reciprocal-square-root estimate refined
+// with one coupled Newton step on (sqrt, 1/(2 sqrt)) and a final residual correction.
+__device__ __forceinline__ double sqrt_nr(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+// K consecutive faces from K+7 points s[0..K+6] (face j uses s[j..j+7]).  The forward
+// differences, the six-fold first differences and the squared third (and shared
+// second) differences are formed once per point and shared by the K faces.  The
+// linear weights and the 1/12 of the candidates are folded into the candidate
+// coefficients, and the weighted sums are evaluated in Horner form over the
+// prefix products of (eps + S_k).  Face j's value is num[j] / den[j].
+template <bool REV, int K, typename Load>
+__device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[K])
+{
+    constexpr int NP = K + 7;
+    // d_k / 12 times the candidate coefficients (candidate k on window points k..k+3)
+    constexpr double q00 = kD0 * (-3.0 / 12.0), q01 = kD0 * (13.0 / 12.0), q02 = kD0 * (-23.0 / 12.0), q03 = kD0 * (25.0 / 12.0);
+    constexpr double q11 = kD1 * (1.0 / 12.0), q12 = kD1 * (-5.0 / 12.0), q13 = kD1 * (13.0 / 12.0), q14 = kD1 * (3.0 / 12.0);
+    constexpr double q22 = kD2 * (-1.0 / 12.0), q23 = kD2 * (7.0 / 12.0), q24 = kD2 * (7.0 / 12.0), q25 = kD2 * (-1.0 / 12.0);
+    constexpr double q33 = kD3 * (3.0 / 12.0), q34 = kD3 * (13.0 / 12.0), q35 = kD3 * (-5.0 / 12.0), q36 = kD3 * (1.0 / 12.0);
+    constexpr double q44 = kD4 * (25.0 / 12.0), q45 = kD4 * (-23.0 / 12.0), q46 = kD4 * (13.0 / 12.0), q47 = kD4 * (-3.0 / 12.0);
+    double s[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) s[i] = ld(i);
+    double D1[NP - 1], D2[NP - 2], D3[NP - 3], G1[NP - 1], C3[NP - 3], B2[NP - 2];
+#pragma unroll
+    for (int i = 0; i < NP - 1; ++i) { D1[i] = s[i + 1] - s[i]; G1[i] = 6.0 * D1[i]; }
+#pragma unroll
+    for (int i = 0; i < NP - 2; ++i) { D2[i] = D1[i + 1] - D1[i]; B2[i] = (kB * D2[i]) * D2[i]; }
+#pragma unroll
+    for (int i = 0; i < NP - 3; ++i) { D3[i] = D2[i + 1] - D2[i]; C3[i] = (kC * D3[i]) * D3[i]; }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        // window point k of this face, and the anchors of candidate k's differences
+        auto w = [&](int k) { return REV ? s[j + 7 - k] : s[j + k]; };
+        auto i1 = [&](int k) { return REV ? j + 6 - k : j + k; };
+        auto i2 = [&](int k) { return REV ? j + 5 - k : j + k; };
+        auto i3 = [&](int k) { return REV ? j + 4 - k : j + k; };
+        const double sb = REV ? -1.0 : 1.0;
+        // first-derivative terms (sign of the whole term is irrelevant: it is squared)
+        const double a0 = fma(11.0, D3[i3(0)], fma(sb * 15.0, D2[i2(0)], G1[i1(0)]));
+        const double a1 = fma(2.0, D3[i3(1)], fma(sb * 9.0, D2[i2(1)], G1[i1(1)]));
+        const double a2 = fma(-1.0, D3[i3(2)], fma(sb * 3.0, D2[i2(2)], G1[i1(2)]));
+        const double a3 = fma(2.0, D3[i3(3)], fma(sb * -3.0, D2[i2(3)], G1[i1(3)]));
+        const double a4 = fma(11.0, D3[i3(4)], fma(sb * -9.0, D2[i2(4)], G1[i1(4)]));
+        // second-derivative terms of candidates 0, 3, 4 (1 and 2 are plain second differences)
+        const double b0 = REV ? fma(-2.0, D3[j + 4], D2[j + 5]) : fma(2.0, D3[j], D2[j]);
+        const double b3 = REV ? D2[j + 2] + D3[j + 1] : D2[j + 3] - D3[j + 3];
+        const double b4 = REV ? fma(2.0, D3[j], D2[j + 1]) : fma(-2.0, D3[j + 4], D2[j + 4]);
+        const double bb1 = REV ? B2[j + 3] : B2[j + 2];
+        const double bb2 = REV ? B2[j + 2] : B2[j + 3];
+        const double S0 = fma(a0, a0, fma(kB * b0, b0, C3[i3(0)]));
+        const double S1 = fma(a1, a1, bb1 + C3[i3(1)]);
+        const double S2 = fma(a2, a2, bb2 + C3[i3(2)]);
+        const double S3 = fma(a3, a3, fma(kB * b3, b3, C3[i3(3)]));
+        double S4 = fma(a4, a4, fma(kB * b4, b4, C3[i3(4)]));
+        S4 = fmax(fmax(fmax(S0, S1), fmax(S2, S3)), S4);
+        const double e0 = kEpsW + S0, e1 = kEpsW + S1, e2 = kEpsW + S2, e3 = kEpsW + S3, e4 = kEpsW + S4;
+        const double c0 = fma(q03, w(3), fma(q02, w(2), fma(q01, w(1), q00 * w(0))));
+        const double c1 = fma(q14, w(4), fma(q13, w(3), fma(q12, w(2), q11 * w(1))));
+        const double c2 = fma(q25, w(5), fma(q24, w(4), fma(q23, w(3), q22 * w(2))));
+        const double c3 = fma(q36, w(6), fma(q35, w(5), fma(q34, w(4), q33 * w(3))));
+        const double c4 = fma(q47, w(7), fma(q46, w(6), fma(q45, w(5), q44 * w(4))));
+        const double p01 = e0 * e1;
+        double N = fma(c0, e1, c1 * e0);
+        double D = fma(kD0, e1, kD1 * e0);
+        N = fma(N, e2, c2 * p01);
+        D = fma(D, e2, kD2 * p01);
+        const double p012 = p01 * e2;
+        N = fma(N, e3, c3 * p012);
+        D = fma(D, e3, kD3 * p012);
+        const double p0123 = p012 * e3;
+        num[j] = fma(N, e4, c4 * p0123);
+        den[j] = fma(D, e4, kD4 * p0123);
+    }
+}
+
 // Steger-Warming split in the form the tiled kernels use: A1..A3 the metric terms,
 // sn = |A|, k1..k3 = A/|A| supplied by the caller (constants when metrics are uniform).
 __device__ __forceinline__ void split_fast(const KParams &k, double d, double u, double v, double w,
@@ -191,9 +287,10 @@ __device__ __forceinline__ void split_fast(const KParams &k, double d, double u,
     const double vs = fma(A3, w, fma(A2, v, A1 * u));
     const double cs = cc * sn;
     const double E1 = vs, E2 = vs - cs, E3 = vs + cs;
-    const double E1P = 0.5 * (E1 + sqrt(fma(E1, E1, k.eps2)));
-    const double E2P = 0.5 * (E2 + sqrt(fma(E2, E2, k.eps2)));
-    const double E3P = 0.5 * (E3 + sqrt(fma(E3, E3, k.eps2)));
+    // eps2 > 0 keeps the radicands positive and normal
+    const double E1P = 0.5 * (E1 + sqrt_nr(fma(E1, E1, k.eps2)));
+    const double E2P = 0.5 * (E2 + sqrt_nr(fma(E2, E2, k.eps2)));
+    const double E3P = 0.5 * (E3 + sqrt_nr(fma(E3, E3, k.eps2)));
     const double E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
     const double t0 = d * k.half_rcp_gamma;
     const double ck1 = cc * k1, ck2 = cc * k2, ck3 = cc * k3;

@@ -42,9 +42,14 @@ struct Ctx {
     // private stream for graph capture (the caller's stream may be the legacy
     // default stream, which cannot be captured); default priority
     hipStream_t capture_stream = nullptr;
+    // tile shape of the tiled FP64 kernels (WS_TILE_*)
+    int tile_shape = WS_TILE_30x16_K2;
 };
 
 KParams make_kparams(const ws_config &cfg);
+
+// tile shape the autotuner found best for these extents, or -1
+int remembered_shape(int nx, int ny, int nz);
 
 // error reporting (thread-local message)
 int set_error(int code, const std::string &msg);
@@ -71,3 +76,6 @@ int launch_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t
 int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream_t s);
 
 }  // namespace ws
+
+// the opaque handle of the C ABI is the internal context
+struct ws_ctx : public ws::Ctx {};
