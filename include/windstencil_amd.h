@@ -51,7 +51,8 @@ extern "C" {
 /* Tile shapes of the tiled FP64 kernels (cells per line x lines per block, faces per thread) */
 #define WS_TILE_15x16_K1 0       /* x: 15x16, y/z: 16x15, one face per thread */
 #define WS_TILE_30x16_K2 1       /* x: 30x16, y/z: 32x15, two faces per thread (default) */
-#define WS_TILE_COUNT 2
+#define WS_TILE_30x16_A16_K2 2   /* x: 30x16, y/z: 32x16 on 16-aligned columns, two faces per thread */
+#define WS_TILE_COUNT 3
 
 /* Ghosted scalar field; layout of cudaField under __HIPCC__ (cuda_commen.h:48-54). */
 typedef struct ws_field {

@@ -15,8 +15,8 @@ LAP = 4
 WS_OK = 0
 WS_PREC_FP64, WS_PREC_MIXED = 0, 1
 WS_KERNEL_TILED, WS_KERNEL_ANCHOR = 0, 1
-WS_TILE_15x16_K1, WS_TILE_30x16_K2 = 0, 1
-TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2)
+WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2 = 0, 1, 2
+TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2)
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32

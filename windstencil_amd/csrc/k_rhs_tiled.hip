@@ -30,23 +30,30 @@ struct Tile {
     static constexpr int F = L + 1;                        // faces per line
     static constexpr int T = (F + K - 1) / K;              // threads per line
     static constexpr int NT = W * T;                       // threads per block
-    static constexpr int PS = DIR == 0 ? NL : W + 1;       // LDS stride: x rows / padded columns
-    static constexpr int QS = DIR == 0 ? W * NL : NL * (W + 1);   // LDS entries per quantity
+    // y/z tiles with 16 columns read one contiguous 128-byte LDS row per quarter-wave
+    // (conflict-free); narrower ones pad the row
+    static constexpr int PS = DIR == 0 ? NL : (W == 16 ? 16 : W + 1);
+    static constexpr int QS = DIR == 0 ? W * NL : NL * PS;   // LDS entries per quantity
     static constexpr int NPTS = W * NL;
+    static constexpr int ROUNDS = (NPTS + NT - 1) / NT;   // split points per thread (upper bound)
+    // y/z tiles of 16 columns start on a multiple of 16 ghosted columns, so that their
+    // rows begin on 128-byte lines of the fields (rows of the reference layout are
+    // line-aligned when the pitch is a multiple of 16 elements)
+    static constexpr int XALIGN = (DIR != 0 && W == 16) ? 16 : 1;
     // the last thread of a line may read up to K+6 points past its first face, i.e.
     // past the line's end; the pad keeps those reads inside the buffer (their values
     // only reach faces that are never stored)
     static constexpr int PAD = (K + 1) * (W + 1) + 8;
-    static_assert(NT <= 256, "block too large");
+    static_assert(NT <= 320, "block too large");
     static_assert(5 * W * F <= 10 * QS, "face buffer must fit in the split buffer");
     // LDS index of point p (0..NL-1) of line l (0..W-1)
-    __device__ static int pidx(int l, int p) { return DIR == 0 ? l * NL + p : p * (W + 1) + l; }
+    __device__ static int pidx(int l, int p) { return DIR == 0 ? l * NL + p : p * PS + l; }
     // face buffer index of face f of line l
     __device__ static int fidx(int l, int f) { return DIR == 0 ? l * F + f : f * W + l; }
 };
 
 template <int DIR, int L, int W, int K, bool FIELDS, bool ACC>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(320)
 k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_metric_fields mf,
            ws_soa du, Box box)
 {
@@ -61,41 +68,65 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
         y0 = box.s[1] + blockIdx.y * W;
         z0 = box.s[2] + blockIdx.z;
     } else if (DIR == 1) {
-        x0 = box.s[0] + blockIdx.x * W;
+        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + blockIdx.x * W;
         y0 = box.s[1] + blockIdx.y * L;
         z0 = box.s[2] + blockIdx.z;
     } else {
-        x0 = box.s[0] + blockIdx.x * W;
+        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + blockIdx.x * W;
         z0 = box.s[2] + blockIdx.y * L;
         y0 = box.s[1] + blockIdx.z;
     }
     // cells of the tile that lie in the box, along the line and across lines
+    // (lines lo_across .. lim_across-1; only the first aligned y/z tile has lo_across > 0)
     const int lim_line = DIR == 0 ? box.e[0] - x0 : DIR == 1 ? box.e[1] - y0 : box.e[2] - z0;
     const int lim_across = DIR == 0 ? box.e[1] - y0 : box.e[0] - x0;
+    const int lo_across = DIR == 0 ? 0 : box.s[0] - x0;
 
     const double cA1 = k.A[3 * DIR + 0], cA2 = k.A[3 * DIR + 1], cA3 = k.A[3 * DIR + 2];
     const double cSn = k.Anorm[DIR];
     const double cK1 = k.Ak[3 * DIR + 0], cK2 = k.Ak[3 * DIR + 1], cK3 = k.Ak[3 * DIR + 2];
 
     // ---- phase 1: split every point of the tile ----------------------------------------
-    for (int pt = t; pt < TL::NPTS; pt += TL::NT) {
+    // all loads of this thread's points are issued before any split arithmetic
+    double ld[TL::ROUNDS][5];
+    int lidx[TL::ROUNDS];
+    bool lok[TL::ROUNDS];
+#pragma unroll
+    for (int r = 0; r < TL::ROUNDS; ++r) {
+        const int pt = t + r * TL::NT;
         int l, p;
         if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }     // x fastest
         else          { l = pt % W;      p = pt / W; }          // x fastest
-        // the line needs points -LAP .. lim_line+LAP-1 and lines 0 .. lim_across-1
-        if (l >= lim_across || p >= lim_line + 2 * LAP) continue;
-        int x, y, z;
-        if (DIR == 0)      { x = x0 - LAP + p; y = y0 + l; z = z0; }
-        else if (DIR == 1) { x = x0 + l; y = y0 - LAP + p; z = z0; }
-        else               { x = x0 + l; y = y0; z = z0 - LAP + p; }
-        const int nyg = k.nyg;
-        const double dd = d.ptr[fidx(d.pitch, nyg, x, y, z)];
-        const double uu = u.ptr[fidx(u.pitch, nyg, x, y, z)];
-        const double vv = v.ptr[fidx(v.pitch, nyg, x, y, z)];
-        const double ww = w.ptr[fidx(w.pitch, nyg, x, y, z)];
-        const double c = cc.ptr[fidx(cc.pitch, nyg, x, y, z)];
+        // the line needs points -LAP .. lim_line+LAP-1 of lines lo_across .. lim_across-1
+        lok[r] = pt < TL::NPTS && l >= lo_across && l < lim_across && p < lim_line + 2 * LAP;
+        lidx[r] = TL::pidx(l, p);
+        if (lok[r]) {
+            int x, y, z;
+            if (DIR == 0)      { x = x0 - LAP + p; y = y0 + l; z = z0; }
+            else if (DIR == 1) { x = x0 + l; y = y0 - LAP + p; z = z0; }
+            else               { x = x0 + l; y = y0; z = z0 - LAP + p; }
+            const int nyg = k.nyg;
+            ld[r][0] = d.ptr[fidx(d.pitch, nyg, x, y, z)];
+            ld[r][1] = u.ptr[fidx(u.pitch, nyg, x, y, z)];
+            ld[r][2] = v.ptr[fidx(v.pitch, nyg, x, y, z)];
+            ld[r][3] = w.ptr[fidx(w.pitch, nyg, x, y, z)];
+            ld[r][4] = cc.ptr[fidx(cc.pitch, nyg, x, y, z)];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < TL::ROUNDS; ++r) {
+        if (!lok[r]) continue;
+        const double dd = ld[r][0], uu = ld[r][1], vv = ld[r][2], ww = ld[r][3], c = ld[r][4];
         double fp[5], fm[5];
         if (FIELDS) {
+            const int pt = t + r * TL::NT;
+            int l, p, x, y, z;
+            if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }
+            else          { l = pt % W;      p = pt / W; }
+            if (DIR == 0)      { x = x0 - LAP + p; y = y0 + l; z = z0; }
+            else if (DIR == 1) { x = x0 + l; y = y0 - LAP + p; z = z0; }
+            else               { x = x0 + l; y = y0; z = z0 - LAP + p; }
+            const int nyg = k.nyg;
             const ws_field &f1 = mf.A[3 * DIR + 0], &f2 = mf.A[3 * DIR + 1], &f3 = mf.A[3 * DIR + 2];
             const double A1 = f1.ptr[fidx(f1.pitch, nyg, x, y, z)];
             const double A2 = f2.ptr[fidx(f2.pitch, nyg, x, y, z)];
@@ -106,7 +137,7 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
         } else {
             split_fast(k, dd, uu, vv, ww, c, cA1, cA2, cA3, cSn, cK1, cK2, cK3, fp, fm);
         }
-        const int li = TL::pidx(l, p);
+        const int li = lidx[r];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             sp[m * TL::QS + li] = fp[m];
@@ -120,11 +151,11 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
     if (DIR == 0) { line = t / TL::T; seg = t % TL::T; }
     else          { line = t % W;     seg = t / W; }
     const int f0 = seg * K;                               // first face of this thread
-    const bool active = (t < TL::NT) && (line < lim_across) && (f0 <= lim_line);
+    const bool active = (t < TL::NT) && (line >= lo_across) && (line < lim_across) && (f0 <= lim_line);
     double Fv[5][K];
     if (active) {
         const int base = TL::pidx(line, f0);
-        const int step = DIR == 0 ? 1 : W + 1;
+        const int step = DIR == 0 ? 1 : TL::PS;
 #pragma unroll 1
         for (int m = 0; m < 5; ++m) {
             const double *qp = sp + m * TL::QS + base;
@@ -160,7 +191,7 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
         int l, i;
         if (DIR == 0) { i = c % L; l = c / L; }
         else          { l = c % W; i = c / W; }
-        if (l >= lim_across || i >= lim_line) continue;
+        if (l < lo_across || l >= lim_across || i >= lim_line) continue;
         int cx, cy, cz;
         if (DIR == 0)      { cx = x0 + i; cy = y0 + l; cz = z0; }
         else if (DIR == 1) { cx = x0 + l; cy = y0 + i; cz = z0; }
@@ -185,10 +216,12 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
 {
     using TL = Tile<DIR, L, W, K>;
     const int nx = box.e[0] - box.s[0], ny = box.e[1] - box.s[1], nz = box.e[2] - box.s[2];
+    // y/z tiles start at the aligned column at or below the box start
+    const int nxa = box.e[0] - box.s[0] / TL::XALIGN * TL::XALIGN;
     dim3 grid;
     if (DIR == 0) grid = dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
-    else if (DIR == 1) grid = dim3((nx + W - 1) / W, (ny + L - 1) / L, nz);
-    else grid = dim3((nx + W - 1) / W, (nz + L - 1) / L, ny);
+    else if (DIR == 1) grid = dim3((nxa + W - 1) / W, (ny + L - 1) / L, nz);
+    else grid = dim3((nxa + W - 1) / W, (nz + L - 1) / L, ny);
     hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC>), grid, dim3(TL::NT), 0, s, k, d, u, v, w, cc, mf,
                        du, box);
 }
@@ -212,8 +245,11 @@ void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field 
         launch_all<15, 16, 16, 15, 1, FIELDS>(k, d, u, v, w, cc, mf, du, box, acc, s);
         break;
     case WS_TILE_30x16_K2:  // x: 30 cells x 16 rows; y, z: 32 cells x 15 columns; two faces per thread
-    default:
         launch_all<30, 16, 32, 15, 2, FIELDS>(k, d, u, v, w, cc, mf, du, box, acc, s);
+        break;
+    case WS_TILE_30x16_A16_K2:  // as above, y/z tiles 16 aligned columns (272 threads)
+    default:
+        launch_all<30, 16, 32, 16, 2, FIELDS>(k, d, u, v, w, cc, mf, du, box, acc, s);
         break;
     }
 }
