@@ -146,6 +146,30 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
     }
     __syncthreads();
 
+    // ---- phase 3 prologue: accumulating launches read du; issue those loads now so
+    // they complete while this thread reconstructs its faces
+    constexpr int CPT = (W * L + TL::NT - 1) / TL::NT;    // cells per thread (upper bound)
+    const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
+    double duo[ACC ? CPT : 1][5];
+    if (ACC) {
+#pragma unroll
+        for (int r = 0; r < CPT; ++r) {
+            const int c = t + r * TL::NT;
+            int l, i;
+            if (DIR == 0) { i = c % L; l = c / L; }
+            else          { l = c % W; i = c / W; }
+            if (c >= W * L || l < lo_across || l >= lim_across || i >= lim_line) continue;
+            int cx, cy, cz;
+            if (DIR == 0)      { cx = x0 + i; cy = y0 + l; cz = z0; }
+            else if (DIR == 1) { cx = x0 + l; cy = y0 + i; cz = z0; }
+            else               { cx = x0 + l; cy = y0; cz = z0 + i; }
+            const double *dp = du.ptr + (size_t)(cx - LAP) +
+                               (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
+#pragma unroll
+            for (int m = 0; m < 5; ++m) duo[r][m] = dp[m * vstride];
+        }
+    }
+
     // ---- phase 2: K faces of one line per thread, all five variables --------------------
     int line, seg;
     if (DIR == 0) { line = t / TL::T; seg = t % TL::T; }
@@ -186,8 +210,10 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
 
     // ---- phase 3: flux difference of each cell ---------------------------------------------
     const double scale0 = -k.rh[DIR];
-    const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
-    for (int c = t; c < W * L; c += TL::NT) {
+#pragma unroll
+    for (int r = 0; r < CPT; ++r) {
+        const int c = t + r * TL::NT;
+        if (c >= W * L) break;
         int l, i;
         if (DIR == 0) { i = c % L; l = c / L; }
         else          { l = c % W; i = c / W; }
@@ -204,7 +230,7 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             const double val = scale * (sp[m * W * TL::F + fr] - sp[m * W * TL::F + fl]);
-            if (ACC) dp[m * vstride] += val;
+            if (ACC) dp[m * vstride] = duo[r][m] + val;
             else dp[m * vstride] = val;
         }
     }
