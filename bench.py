@@ -5,9 +5,12 @@ A step is one full RHS evaluation L(U) of the 600x256x128 periodic box
 (the Sod-plus-noise workload): on-device ghost fill of rho,u,v,w,T, sound
 speed over the ghosted box, and the three directional WENO7-SYMBO /
 Steger-Warming kernels writing du.  Inputs are resident in HBM before the
-timed region.  Prints one JSON line.
+timed region.  The headline precision (--precision, default FP64) gives `value`;
+the other precision is timed on the same state in the same run and reported
+under "other_precision" with its speedup and its RHS error against the first.
+Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed] [--no-second] [--no-cpu]
 """
 import argparse
 import ctypes
@@ -38,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline leg")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-second", action="store_true",
+                    help="skip timing the other precision (mixed when --precision fp64)")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape (default: autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for HBM counter calibration)")
@@ -136,38 +141,58 @@ def main():
         if ev is not None:
             ev[2].record(stream)
 
-    rhs()
-    torch.cuda.synchronize()
-    tune = None
-    if args.tile >= 0:
-        W.check(L.ws_ctx_set_tile_shape(ctx.handle, args.tile))
-    else:
-        times = (ctypes.c_float * len(W.TILE_SHAPES))()
-        W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 5, times, sh))
-        tune = {str(s): round(times[s], 4) for s in W.TILE_SHAPES}
-    chosen = ctypes.c_int32()
-    W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
-    for _ in range(args.warmup):
+    def run(precision):
+        """Autotune (unless a shape is forced), warm up, then time args.steps RHS
+        evaluations: wall clock over the whole loop, and HIP events per phase."""
+        ctx.set_precision(precision)
         rhs()
-    torch.cuda.synchronize()
-    # whole-job timing
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rhs()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    # per-phase timing with HIP events on the launch stream
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    for k in range(args.steps):
-        rhs(evs[k])
-    torch.cuda.synchronize()
-    prep_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
-    kern_ms = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
+        torch.cuda.synchronize()
+        tune = None
+        if args.tile >= 0:
+            W.check(L.ws_ctx_set_tile_shape(ctx.handle, args.tile))
+        else:
+            times = (ctypes.c_float * len(W.TILE_SHAPES))()
+            W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 5, times, sh))
+            tune = {str(s): round(times[s], 4) for s in W.TILE_SHAPES}
+        chosen = ctypes.c_int32()
+        W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
+        for _ in range(args.warmup):
+            rhs()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rhs()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+        for k in range(args.steps):
+            rhs(evs[k])
+        torch.cuda.synchronize()
+        prep = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
+        kern = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
+        return wall / args.steps * 1e3, prep, kern, chosen.value, tune
 
-    ms = wall / args.steps * 1e3
+    other = W.WS_PREC_MIXED if prec == W.WS_PREC_FP64 else W.WS_PREC_FP64
+    second = None
+    if not args.no_second:
+        # the other precision on the same state, for the FP64-vs-mixed comparison
+        ms2, _, kern2, shape2, tune2 = run(other)
+        du2 = st.interior("du").clone()
+        second = {"precision": "mixed" if other == W.WS_PREC_MIXED else "fp64",
+                  "value": round(N / (ms2 * 1e-3) / 1e6, 2), "ms_per_step": round(ms2, 4),
+                  "kernel_ms": round(kern2, 4), "tile_shape": shape2, "autotune_ms": tune2}
+    ms, prep_ms, kern_ms, shape, tune = run(prec)
     value = N / (ms * 1e-3) / 1e6
     flops_cell, flops_dir = algo_flops()
     achieved = flops_cell * N / (kern_ms * 1e-3) / 1e12
+    if second is not None:
+        du1 = st.interior("du")
+        second["rel_l2_vs_" + args.precision] = float(torch.linalg.vector_norm(du2 - du1) /
+                                                      torch.linalg.vector_norm(du1))
+        mixed_ms = ms if prec == W.WS_PREC_MIXED else second["ms_per_step"]
+        fp64_ms = ms if prec == W.WS_PREC_FP64 else second["ms_per_step"]
+        second["speedup_mixed_over_fp64"] = round(fp64_ms / mixed_ms, 4)
+        del du2
 
     parity = None
     if not args.no_parity:
@@ -208,8 +233,9 @@ def main():
                      "kernel": "k_rhs_tile x3 directions (one RHS)",
                      "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
         "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
-        "tile_shape": chosen.value, "autotune_ms": tune,
+        "tile_shape": shape, "autotune_ms": tune,
         "parity_rel_l2_vs_anchor": parity,
+        "other_precision": second,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prims, nx, ny, nz, args.cpu_seconds)

@@ -39,7 +39,11 @@ extern "C" {
 #define WS_ERR_HIP (-3)          /* a HIP runtime call failed */
 #define WS_ERR_NOMEM (-4)        /* device allocation failed */
 
-/* precision of the RHS arithmetic */
+/* precision of the RHS arithmetic.  MIXED: splitting, merge and flux difference in FP64;
+ * WENO7 smoothness, weights and candidates in FP32 on first differences of the split
+ * fluxes taken relative to an FP64 base point per face (error ~1e-7 of the local flux
+ * variation; the FP32 weight products hold for flux differences up to ~1e3).  Applies
+ * to the tiled kernels; the anchor kernels are always FP64. */
 #define WS_PREC_FP64 0
 #define WS_PREC_MIXED 1
 
@@ -193,9 +197,9 @@ int ws_mem_value_init(double value, double *ptr, uint32_t pitch, uint32_t size_x
  * double).  No counterpart: the reference uses a fixed DT. */
 int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipStream_t stream);
 
-/* Time every tile shape of the tiled FP64 kernels on this state (two untimed and `reps`
- * timed RHS launches each, HIP events on `stream`), keep the fastest for the context and
- * remember it for contexts of the same extents.  times_ms (WS_TILE_COUNT entries) may be
+/* Time every tile shape of the tiled kernels, in the context's precision, on this state
+ * (two untimed and `reps` timed RHS launches each, HIP events on `stream`), keep the
+ * fastest for the context and remember it for contexts of the same extents and precision.  times_ms (WS_TILE_COUNT entries) may be
  * NULL.  Overwrites st->du; synchronises the stream.  No counterpart in the reference. */
 int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
 

@@ -1,9 +1,10 @@
-// autotune.cpp -- pick the tile shape of the tiled FP64 kernels by timing them.
+// autotune.cpp -- pick the tile shape of the tiled kernels by timing them.
 //
-// Every candidate shape runs the full inviscid RHS on the caller's state: two
-// untimed launches, then `reps` launches bracketed by HIP events on the caller's
-// stream.  The fastest shape becomes the context's shape and is remembered for the
-// grid extents, so later contexts of the same size start from it.
+// Every candidate shape runs the full inviscid RHS on the caller's state, in the
+// context's precision: two untimed launches, then `reps` launches bracketed by HIP
+// events on the caller's stream.  The fastest shape becomes the context's shape and
+// is remembered for the grid extents and precision, so later contexts of the same
+// size and precision start from it.
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -17,16 +18,23 @@ using namespace ws;
 namespace {
 
 std::mutex g_mu;
-std::map<std::tuple<int, int, int>, int> g_best;   // (nx, ny, nz) -> shape
+std::map<std::tuple<int, int, int, int>, int> g_best;   // (nx, ny, nz, precision) -> shape
+
+int launch(const Ctx &c, const ws_state *st, const ws_job &job, hipStream_t s)
+{
+    if (c.cfg.precision == WS_PREC_MIXED)
+        return launch_rhs_mixed(c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, job, 0, s);
+    return launch_rhs_tiled(c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, job, 0, s);
+}
 
 }  // namespace
 
 namespace ws {
 
-int remembered_shape(int nx, int ny, int nz)
+int remembered_shape(int nx, int ny, int nz, int precision)
 {
     std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_best.find(std::make_tuple(nx, ny, nz));
+    auto it = g_best.find(std::make_tuple(nx, ny, nz, precision));
     return it == g_best.end() ? -1 : it->second;
 }
 
@@ -49,11 +57,11 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
     for (int shape = 0; shape < WS_TILE_COUNT && !r; ++shape) {
         c->tile_shape = shape;
         for (int i = 0; i < 2 && !r; ++i)
-            r = launch_rhs_tiled(*c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, job, 0, s);
+            r = launch(*c, st, job, s);
         if (r) break;
         hipEventRecord(e0, s);
         for (int i = 0; i < reps && !r; ++i)
-            r = launch_rhs_tiled(*c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, job, 0, s);
+            r = launch(*c, st, job, s);
         hipEventRecord(e1, s);
         if (r) break;
         e = hipEventSynchronize(e1);
@@ -69,7 +77,7 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
     c->tile_shape = r ? saved : best;
     if (!r) {
         std::lock_guard<std::mutex> lk(g_mu);
-        g_best[std::make_tuple(c->cfg.nx, c->cfg.ny, c->cfg.nz)] = best;
+        g_best[std::make_tuple(c->cfg.nx, c->cfg.ny, c->cfg.nz, c->cfg.precision)] = best;
     }
     return r;
 }

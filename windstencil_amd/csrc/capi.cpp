@@ -221,7 +221,7 @@ int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
     if (!c) return set_error(WS_ERR_NOMEM, "host allocation failed");
     c->cfg = *cfg;
     c->kp = make_kparams(*cfg);
-    const int shape = remembered_shape(cfg->nx, cfg->ny, cfg->nz);
+    const int shape = remembered_shape(cfg->nx, cfg->ny, cfg->nz, cfg->precision);
     if (shape >= 0) c->tile_shape = shape;
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return hip_error(e, "hipGetDevice"); }
@@ -255,6 +255,8 @@ int ws_ctx_set_precision(ws_ctx *c, int32_t precision)
     if (!c) return set_error(WS_ERR_INVALID, "null context");
     if (precision != WS_PREC_FP64 && precision != WS_PREC_MIXED) return set_error(WS_ERR_INVALID, "unknown precision");
     c->cfg.precision = precision;
+    const int shape = remembered_shape(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision);
+    if (shape >= 0) c->tile_shape = shape;
     return WS_OK;
 }
 

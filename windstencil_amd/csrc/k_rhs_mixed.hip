@@ -1,16 +1,14 @@
-// k_rhs_mixed.hip -- mixed-precision inviscid RHS (FP32 splitting and candidate
-// stencils, FP64 weights near discontinuities, FP64 flux difference).
-#include <hip/hip_runtime.h>
-
-#include "ws_device.h"
+// k_rhs_mixed.hip -- mixed-precision inviscid RHS: FP64 splitting, WENO7 weights and
+// candidates in FP32 on first differences taken relative to FP64 base points, FP64
+// merge and flux difference.  The kernel template is in rhs_tile.h.
+#include "rhs_tile.h"
 
 namespace ws {
 
 int launch_rhs_mixed(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s)
 {
-    (void)c; (void)d; (void)u; (void)v; (void)w; (void)cc; (void)m; (void)du; (void)job; (void)accumulate; (void)s;
-    return set_error(WS_ERR_UNSUPPORTED, "mixed-precision RHS is not built yet");
+    return launch_rhs_tile<true>(c, d, u, v, w, cc, m, du, job, accumulate, s);
 }
 
 }  // namespace ws

@@ -278,6 +278,80 @@ __device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[
     }
 }
 
+// Mixed-precision form of weno_k: the window stays FP64 in LDS; its first differences
+// are formed in FP64 and rounded to FP32, and everything after that (second and third
+// differences, smoothness, weights, candidates) runs in FP32 on values taken relative to
+// the face's base point w3.  The face value is base[j] + corr[j]: base in FP64, the FP32
+// correction carries an error relative to the local variation of the window, not to the
+// size of the flux.
+template <bool REV, int K, typename Load>
+__device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (&corr)[K])
+{
+    constexpr int NP = K + 7;
+    constexpr float fD0 = (float)kD0, fD1 = (float)kD1, fD2 = (float)kD2, fD3 = (float)kD3, fD4 = (float)kD4;
+    constexpr float fB = 39.0f, fC = 39.05f, fEps = 3.6e-9f;
+    double s[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) s[i] = ld(i);
+    float D1[NP - 1], D2[NP - 2], D3[NP - 3], G1[NP - 1], C3[NP - 3], B2[NP - 2];
+#pragma unroll
+    for (int i = 0; i < NP - 1; ++i) { D1[i] = (float)(s[i + 1] - s[i]); G1[i] = 6.0f * D1[i]; }
+#pragma unroll
+    for (int i = 0; i < NP - 2; ++i) { D2[i] = D1[i + 1] - D1[i]; B2[i] = (fB * D2[i]) * D2[i]; }
+#pragma unroll
+    for (int i = 0; i < NP - 3; ++i) { D3[i] = D2[i + 1] - D2[i]; C3[i] = (fC * D3[i]) * D3[i]; }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        auto i1 = [&](int k) { return REV ? j + 6 - k : j + k; };
+        auto i2 = [&](int k) { return REV ? j + 5 - k : j + k; };
+        auto i3 = [&](int k) { return REV ? j + 4 - k : j + k; };
+        const float sb = REV ? -1.0f : 1.0f;
+        const float a0 = fmaf(11.0f, D3[i3(0)], fmaf(sb * 15.0f, D2[i2(0)], G1[i1(0)]));
+        const float a1 = fmaf(2.0f, D3[i3(1)], fmaf(sb * 9.0f, D2[i2(1)], G1[i1(1)]));
+        const float a2 = fmaf(-1.0f, D3[i3(2)], fmaf(sb * 3.0f, D2[i2(2)], G1[i1(2)]));
+        const float a3 = fmaf(2.0f, D3[i3(3)], fmaf(sb * -3.0f, D2[i2(3)], G1[i1(3)]));
+        const float a4 = fmaf(11.0f, D3[i3(4)], fmaf(sb * -9.0f, D2[i2(4)], G1[i1(4)]));
+        const float b0 = REV ? fmaf(-2.0f, D3[j + 4], D2[j + 5]) : fmaf(2.0f, D3[j], D2[j]);
+        const float b3 = REV ? D2[j + 2] + D3[j + 1] : D2[j + 3] - D3[j + 3];
+        const float b4 = REV ? fmaf(2.0f, D3[j], D2[j + 1]) : fmaf(-2.0f, D3[j + 4], D2[j + 4]);
+        const float bb1 = REV ? B2[j + 3] : B2[j + 2];
+        const float bb2 = REV ? B2[j + 2] : B2[j + 3];
+        const float S0 = fmaf(a0, a0, fmaf(fB * b0, b0, C3[i3(0)]));
+        const float S1 = fmaf(a1, a1, bb1 + C3[i3(1)]);
+        const float S2 = fmaf(a2, a2, bb2 + C3[i3(2)]);
+        const float S3 = fmaf(a3, a3, fmaf(fB * b3, b3, C3[i3(3)]));
+        float S4 = fmaf(a4, a4, fmaf(fB * b4, b4, C3[i3(4)]));
+        S4 = fmaxf(fmaxf(fmaxf(S0, S1), fmaxf(S2, S3)), S4);
+        const float e0 = fEps + S0, e1 = fEps + S1, e2 = fEps + S2, e3 = fEps + S3, e4 = fEps + S4;
+        // window relative to its base point w3: dw(k) = w(k) - w(3), from the FP32 differences
+        // (P: w(k) = s[j+k], so w(k+1) - w(k) = D1[j+k]; M: w(k) = s[j+7-k], w(k+1) - w(k) = -D1[j+6-k])
+        auto dw1 = [&](int k) { return REV ? -D1[j + 6 - k] : D1[j + k]; };  // w(k+1) - w(k)
+        const float r4 = dw1(3), r5 = r4 + dw1(4), r6 = r5 + dw1(5), r7 = r6 + dw1(6);
+        const float r2 = -dw1(2), r1 = r2 - dw1(1), r0 = r1 - dw1(0);
+        // candidates on the relative window (coefficients of each candidate sum to 12, so the
+        // base drops out), linear weights and 1/12 folded in
+        const float c0 = fmaf(fD0 * (13.0f / 12.0f), r1, fmaf(fD0 * (-23.0f / 12.0f), r2, fD0 * (-3.0f / 12.0f) * r0));
+        const float c1 = fmaf(fD1 * (3.0f / 12.0f), r4, fmaf(fD1 * (-5.0f / 12.0f), r2, fD1 * (1.0f / 12.0f) * r1));
+        const float c2 = fmaf(fD2 * (7.0f / 12.0f), r4, fmaf(fD2 * (-1.0f / 12.0f), r5, fD2 * (-1.0f / 12.0f) * r2));
+        const float c3 = fmaf(fD3 * (13.0f / 12.0f), r4, fmaf(fD3 * (-5.0f / 12.0f), r5, fD3 * (1.0f / 12.0f) * r6));
+        const float c4 = fmaf(fD4 * (25.0f / 12.0f), r4, fmaf(fD4 * (-23.0f / 12.0f), r5,
+                              fmaf(fD4 * (13.0f / 12.0f), r6, fD4 * (-3.0f / 12.0f) * r7)));
+        const float p01 = e0 * e1;
+        float N = fmaf(c0, e1, c1 * e0);
+        float D = fmaf(fD0, e1, fD1 * e0);
+        N = fmaf(N, e2, c2 * p01);
+        D = fmaf(D, e2, fD2 * p01);
+        const float p012 = p01 * e2;
+        N = fmaf(N, e3, c3 * p012);
+        D = fmaf(D, e3, fD3 * p012);
+        const float p0123 = p012 * e3;
+        N = fmaf(N, e4, c4 * p0123);
+        D = fmaf(D, e4, fD4 * p0123);
+        corr[j] = N * __builtin_amdgcn_rcpf(D);
+        base[j] = REV ? s[j + 4] : s[j + 3];
+    }
+}
+
 // Steger-Warming split in the form the tiled kernels use: A1..A3 the metric terms,
 // sn = |A|, k1..k3 = A/|A| supplied by the caller (constants when metrics are uniform).
 __device__ __forceinline__ void split_fast(const KParams &k, double d, double u, double v, double w,

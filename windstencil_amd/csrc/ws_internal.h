@@ -42,14 +42,14 @@ struct Ctx {
     // private stream for graph capture (the caller's stream may be the legacy
     // default stream, which cannot be captured); default priority
     hipStream_t capture_stream = nullptr;
-    // tile shape of the tiled FP64 kernels (WS_TILE_*)
+    // tile shape of the tiled kernels, FP64 and mixed (WS_TILE_*)
     int tile_shape = WS_TILE_30x16_K2;
 };
 
 KParams make_kparams(const ws_config &cfg);
 
-// tile shape the autotuner found best for these extents, or -1
-int remembered_shape(int nx, int ny, int nz);
+// tile shape the autotuner found best for these extents and precision, or -1
+int remembered_shape(int nx, int ny, int nz, int precision);
 
 // error reporting (thread-local message)
 int set_error(int code, const std::string &msg);
