@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--no-second", action="store_true",
                     help="skip timing the other precision (mixed when --precision fp64)")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape (default: autotune)")
+    ap.add_argument("--workload", default="rhs", choices=["rhs", "batched", "rk3"],
+                    help="rhs: the headline RHS line (default); batched: 48 uneven blocks; "
+                         "rk3: SSP-RK3 steps of one box in a graph")
+    ap.add_argument("--rk3-steps", type=int, default=0, help="RK3 steps of the batched/rk3 workloads "
+                                                            "(default: 10 batched, 100 rk3)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for HBM counter calibration)")
     return ap.parse_args()
@@ -101,6 +106,137 @@ def cpu_baseline(prims, nx, ny, nz, budget_s):
                       f"C restatement -O3 -fopenmp, {threads} threads"}
 
 
+def _common(args, metric, value, ms, dtype, config, **extra):
+    out = {"metric": metric, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def bench_batched(args):
+    """48 periodic blocks with extents drawn from U{32..256} (seed 1), each a freestream
+    of Mach U[0.3, 3] in a seeded direction with Gaussian blobs.  A step is one RHS of
+    every block (per-block ghost fill and sound speed, then the batched persistent RHS
+    launches); then RK3 steps of the whole batch replayed from one hipGraph per step."""
+    import torch
+    import windstencil_amd as W
+    from windstencil_amd import initial as I
+    L = W.lib()
+    prec = W.WS_PREC_FP64 if args.precision == "fp64" else W.WS_PREC_MIXED
+    extents = I.multiblock_draw(1)
+    stream = torch.cuda.current_stream()
+    sh = W.stream_handle(stream)
+    states = []
+    for i, (nx, ny, nz) in enumerate(extents):
+        st = W.TorchState(nx, ny, nz)
+        st.set_prims(*I.block_freestream(nx, ny, nz, 1000 + i))
+        ctx = W.Context(W.default_config(nx, ny, nz))
+        W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                                 W.job((0, 0, 0), (nx, ny, nz)), sh))
+        states.append(st)
+    b = W.Batch(W.default_config(8, 8, 8, precision=prec), extents, states)
+    N = b.cells()
+    W.check(L.ws_batch_du_comput(b.handle, sh))
+    torch.cuda.synchronize()
+    tune = None
+    if args.tile >= 0:
+        W.check(L.ws_batch_set_tile_shape(b.handle, args.tile))
+    else:
+        times = (ctypes.c_float * len(W.TILE_SHAPES))()
+        W.check(L.ws_batch_autotune(b.handle, 3, times, sh))
+        tune = {str(s): round(times[s], 4) for s in W.TILE_SHAPES}
+    chosen = ctypes.c_int32()
+    W.check(L.ws_batch_get_tile_shape(b.handle, ctypes.byref(chosen)))
+    for _ in range(args.warmup):
+        W.check(L.ws_batch_du_comput(b.handle, sh))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        W.check(L.ws_batch_du_comput(b.handle, sh))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    # the batched RHS launches alone (inputs already ghost-filled), HIP events on the stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        W.check(L.ws_batch_inviscid_rhs(b.handle, sh))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kern_ms = e0.elapsed_time(e1) / args.steps
+    flops_cell, _ = algo_flops()
+    achieved = flops_cell * N / (kern_ms * 1e-3) / 1e12
+    nsteps = args.rk3_steps or 10
+    W.check(L.ws_batch_rk3_run(b.handle, 1, 0.5, 0, sh))      # warm-up step
+    t0 = time.perf_counter()
+    W.check(L.ws_batch_rk3_run(b.handle, nsteps, 0.5, 1, sh))
+    rk3_s = time.perf_counter() - t0
+    out = _common(args, "batched inviscid RHS Mcell-updates/s, 48 uneven periodic blocks", N / (ms * 1e-3) / 1e6,
+                  ms, "f64" if prec == W.WS_PREC_FP64 else "f64-state/mixed-rhs",
+                  {"workload": "48 periodic blocks, extents U{32..256} (seed 1), Mach U[0.3,3] freestream + blobs",
+                   "blocks": len(extents), "cells": N, "precision": args.precision},
+                  roofline={"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                            "kernel": "k_rhs_batch x3 directions (one RHS of every block)",
+                            "kernel_ms": round(kern_ms, 4)},
+                  tile_shape=chosen.value, autotune_ms=tune,
+                  rk3={"steps": nsteps, "graph": True, "seconds": round(rk3_s, 4),
+                       "mcell_updates_per_s": round(N * 3 * nsteps / rk3_s / 1e6, 2),
+                       "note": "N x 3 stages x steps / wall time; CFL 0.5 from the per-block minimum each step"})
+    print(json.dumps(out))
+
+
+def bench_rk3(args):
+    """SSP-RK3 steps of one periodic box, replayed from one hipGraph per step (dt from
+    the CFL reduction every step): the FP64 drop-in workload (600x128x128 vortex
+    lattice, 100 steps) or, with --grid 1200x256x256, Mach 2 blobs with a Mach 2 planar
+    shock (1000 steps)."""
+    import torch
+    import windstencil_amd as W
+    from windstencil_amd import initial as I
+    L = W.lib()
+    prec = W.WS_PREC_FP64 if args.precision == "fp64" else W.WS_PREC_MIXED
+    grid = args.grid if args.grid != "600x256x128" else "600x128x128"
+    nx, ny, nz = (int(v) for v in grid.split("x"))
+    N = nx * ny * nz
+    if (nx, ny, nz) == (600, 128, 128):
+        prims, what, nsteps = I.vortex_lattice(nx, ny, nz, 2024), "six Yee vortices (beta 5) on a Mach 0.5 stream", 100
+    else:
+        prims, what, nsteps = I.blobs_with_shock(nx, ny, nz, 7), "Mach 2 blobs with a Mach 2 planar shock", 1000
+    nsteps = args.rk3_steps or nsteps
+    stream = torch.cuda.current_stream()
+    sh = W.stream_handle(stream)
+    ctx = W.Context(W.default_config(nx, ny, nz, precision=prec))
+    st = W.TorchState(nx, ny, nz)
+    st.set_prims(*prims)
+    W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                             W.job((0, 0, 0), (nx, ny, nz)), sh))
+    if args.tile >= 0:
+        W.check(L.ws_ctx_set_tile_shape(ctx.handle, args.tile))
+    else:
+        W.check(L.ws_du_comput(ctx.handle, ctypes.byref(st.ws), None, sh))
+        W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 3, None, sh))
+    W.check(L.ws_rk3_run(ctx.handle, ctypes.byref(st.ws), 2, 0.5, 1, sh))     # warm-up steps
+    st.sync_from_ws()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    W.check(L.ws_rk3_run(ctx.handle, ctypes.byref(st.ws), nsteps, 0.5, 1, sh))
+    secs = time.perf_counter() - t0
+    st.sync_from_ws()
+    f = st.interior("f")
+    finite = bool(torch.isfinite(f).all())
+    ms = secs / nsteps * 1e3
+    out = _common(args, f"SSP-RK3 Mcell-updates/s (N x 3 stages x steps / time), {nx}x{ny}x{nz}",
+                  N * 3 * nsteps / secs / 1e6, ms, "f64" if prec == W.WS_PREC_FP64 else "f64-state/mixed-rhs",
+                  {"workload": f"{nsteps} SSP-RK3 steps of the {nx}x{ny}x{nz} periodic box ({what}), CFL 0.5, "
+                               "one hipGraph per step", "grid": [nx, ny, nz], "precision": args.precision,
+                   "cells": N},
+                  rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
+                  totals=[float(v) for v in f.sum(dim=(1, 2, 3)).cpu()])
+    out["steps"] = nsteps
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -113,6 +249,12 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "batched":
+        bench_batched(args)
+        return
+    if args.workload == "rk3":
+        bench_rk3(args)
         return
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
     N = nx * ny * nz

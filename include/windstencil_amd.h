@@ -199,8 +199,9 @@ int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipSt
 
 /* Time every tile shape of the tiled kernels, in the context's precision, on this state
  * (two untimed and `reps` timed RHS launches each, HIP events on `stream`), keep the
- * fastest for the context and remember it for contexts of the same extents and precision.  times_ms (WS_TILE_COUNT entries) may be
- * NULL.  Overwrites st->du; synchronises the stream.  No counterpart in the reference. */
+ * fastest for the context and remember it for contexts of the same extents and precision.
+ * times_ms (WS_TILE_COUNT entries) may be NULL.  Overwrites st->du; synchronises the
+ * stream.  No counterpart in the reference. */
 int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
 
 /* ---- time integration ---------------------------------------------------- */
@@ -215,6 +216,43 @@ int ws_rk3_step(ws_ctx *ctx, ws_state *st, const double *dt_dev, hipStream_t str
  * and replays it.  Synchronises the stream before returning. */
 int ws_rk3_run(ws_ctx *ctx, ws_state *st, int32_t nsteps, double cfl, int32_t use_graph,
                hipStream_t stream);
+
+/* ---- batches of blocks ------------------------------------------------------
+ * Many independent periodic blocks of uneven size advanced together.  No counterpart
+ * in the reference, which runs one block per process (its multi-block runs are MPI
+ * decompositions of one grid).  Every block has the batch's physics, metrics and
+ * precision, its own extents, spacing h = 1/(n-1) per direction (the reference's
+ * convention, src/runtime/OCFD_init.cpp) and a caller-owned state.  The RHS of all
+ * blocks is one persistent launch per direction over a tile list; one dt (the minimum
+ * of the blocks' CFL steps) advances every block. */
+typedef struct ws_batch ws_batch;
+
+/* extents: 3*nblocks (nx, ny, nz per block); states: nblocks states whose buffers stay
+ * owned by the caller and must outlive the batch.  base supplies everything else
+ * (its nx, ny, nz, hx, hy, hz are ignored). */
+int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *extents, const ws_state *states,
+                    ws_batch **out);
+int ws_batch_destroy(ws_batch *batch);
+/* Block i's state as the batch holds it now (RK3 steps exchange f and fn). */
+int ws_batch_block(const ws_batch *batch, int32_t i, ws_state *st);
+int64_t ws_batch_cells(const ws_batch *batch);
+int ws_batch_set_precision(ws_batch *batch, int32_t precision);
+int ws_batch_set_kernel(ws_batch *batch, int32_t kernel);
+int ws_batch_set_tile_shape(ws_batch *batch, int32_t shape);
+int ws_batch_get_tile_shape(const ws_batch *batch, int32_t *shape);
+/* RHS of every block from ghost-filled d,u,v,w and cc (overwrites du). */
+int ws_batch_inviscid_rhs(ws_batch *batch, hipStream_t stream);
+/* ws_du_comput for every block: ghost fill, sound speed, then the batched RHS. */
+int ws_batch_du_comput(ws_batch *batch, hipStream_t stream);
+/* *dt_dev = min over blocks of cfl*min(h)/max(|u_d| + c). */
+int ws_batch_cfl_dt(ws_batch *batch, double cfl, double *dt_dev, hipStream_t stream);
+/* One SSP-RK3 step of every block (as ws_rk3_step). */
+int ws_batch_rk3_step(ws_batch *batch, const double *dt_dev, hipStream_t stream);
+/* nsteps batch RK3 steps, dt from the CFL reduction (cfl > 0) or cfg.dt; use_graph = 1
+ * captures a whole step (CFL included) in one hipGraph and replays it. */
+int ws_batch_rk3_run(ws_batch *batch, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t stream);
+/* Time every tile shape of the batched RHS and keep the fastest (as ws_autotune). */
+int ws_batch_autotune(ws_batch *batch, int32_t reps, float *times_ms, hipStream_t stream);
 
 #ifdef __cplusplus
 }

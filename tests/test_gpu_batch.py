@@ -1,0 +1,158 @@
+"""GPU tests of batches of uneven blocks (one persistent RHS launch per direction).
+
+The batched kernels run the same per-tile arithmetic as the single-block kernels, so
+a batch must reproduce the per-block results bit for bit: the RHS for every tile
+shape and both precisions, the RK3 state at a fixed dt, the CFL step (the minimum of
+the blocks'), and graph replay against eager steps.  Against the oracle the batch
+meets the single-block bars (1e-12 relative L2 for FP64, bitwise for the anchor
+kernels).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import windstencil_amd as W
+from windstencil_amd.initial import SplitMix64
+from tests.gpu_util import gpu_du_comput, log, make_ctx, oracle_du, upload_state
+from tests.helpers import random_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ["blobs", "vortex", "sod", "shock", "rough"]
+
+
+def _extents(seed, n, lo, hi):
+    rng = SplitMix64(seed)
+    return [(rng.randint(lo, hi), rng.randint(lo, hi), rng.randint(lo, hi)) for _ in range(n)]
+
+
+def _batch(extents, seed, precision=W.WS_PREC_FP64, kernel=W.WS_KERNEL_TILED, **kw):
+    prims = [random_state(*e, seed + i, KINDS[i % len(KINDS)]) for i, e in enumerate(extents)]
+    states = [upload_state(p) for p in prims]
+    base = W.default_config(8, 8, 8, precision=precision, kernel=kernel, **kw)
+    return W.Batch(base, extents, states), prims
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _du(st):
+    return st.interior("du").cpu().numpy()
+
+
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+@pytest.mark.parametrize("tile", W.TILE_SHAPES)
+def test_batch_rhs_equals_single_block_runs(ws, tile, precision):
+    extents = _extents(7, 9, 5, 47)
+    log(f"batch rhs tile {tile} precision {precision}: {extents}")
+    b, prims = _batch(extents, 100, precision=precision)
+    W.check(W.lib().ws_batch_set_tile_shape(b.handle, tile))
+    W.check(W.lib().ws_batch_du_comput(b.handle, W.stream_handle()))
+    _sync()
+    for e, p, st in zip(extents, prims, b.states):
+        ctx = make_ctx(*e, precision=precision)
+        ctx.set_tile_shape(tile)
+        ref = gpu_du_comput(ctx, upload_state(p))
+        assert np.array_equal(_du(st), ref), e
+
+
+def test_batch_rhs_against_oracle(orc, ws):
+    extents = [(37, 21, 13), (16, 40, 9), (24, 16, 30)]
+    b, prims = _batch(extents, 200)
+    W.check(W.lib().ws_batch_du_comput(b.handle, W.stream_handle()))
+    _sync()
+    for e, p, st in zip(extents, prims, b.states):
+        err = rel_l2(_du(st), oracle_du(orc, p))
+        log(f"  {e}: rel L2 {err:.3e}")
+        assert err <= 1e-12
+
+
+def test_batch_anchor_kernels_bitwise_oracle(orc, ws):
+    extents = [(20, 12, 9), (11, 17, 14)]
+    b, prims = _batch(extents, 300, kernel=W.WS_KERNEL_ANCHOR)
+    W.check(W.lib().ws_batch_du_comput(b.handle, W.stream_handle()))
+    _sync()
+    for p, st in zip(prims, b.states):
+        assert np.array_equal(_du(st), oracle_du(orc, p))
+
+
+def test_batch_cfl_is_min_over_blocks(ws):
+    import torch
+    extents = _extents(11, 6, 6, 30)
+    b, prims = _batch(extents, 400)
+    dt = torch.zeros(1, dtype=torch.float64, device="cuda")
+    s = W.stream_handle()
+    W.check(W.lib().ws_batch_cfl_dt(b.handle, 0.5, ctypes.c_void_p(dt.data_ptr()), s))
+    each = torch.zeros(len(extents), dtype=torch.float64, device="cuda")
+    for i, (e, st) in enumerate(zip(extents, b.states)):
+        ctx = make_ctx(*e)
+        W.check(W.lib().ws_cfl_dt(ctx.handle, ctypes.byref(st.ws), 0.5,
+                                  ctypes.c_void_p(each.data_ptr() + 8 * i), s))
+    _sync()
+    assert float(dt[0]) == float(each.min())
+
+
+def _to_cons(ctx, st):
+    L = W.lib()
+    nx, ny, nz = st.n
+    W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                             W.job((0, 0, 0), (nx, ny, nz)), W.stream_handle()))
+
+
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision):
+    extents = _extents(13, 5, 6, 33)
+    dt = 2e-4
+    b, prims = _batch(extents, 500, precision=precision, dt=dt)
+    for e, st in zip(extents, b.states):
+        _to_cons(make_ctx(*e), st)
+    log(f"batch rk3 fixed dt {extents}")
+    W.check(W.lib().ws_batch_rk3_run(b.handle, 6, -1.0, 0, W.stream_handle()))
+    b.sync_states()
+    for e, p, st in zip(extents, prims, b.states):
+        ctx = make_ctx(*e, precision=precision, dt=dt)
+        ref = upload_state(p)
+        _to_cons(ctx, ref)
+        W.check(W.lib().ws_rk3_run(ctx.handle, ctypes.byref(ref.ws), 6, -1.0, 0, W.stream_handle()))
+        ref.sync_from_ws()
+        _sync()
+        assert np.array_equal(st.interior("f").cpu().numpy(), ref.interior("f").cpu().numpy()), e
+
+
+@pytest.mark.parametrize("nsteps", [2, 3, 5])
+def test_batch_graph_replay_equals_eager(ws, nsteps):
+    extents = _extents(17, 4, 6, 28)
+    runs = []
+    for graph in (1, 0):
+        b, _ = _batch(extents, 600)
+        for e, st in zip(extents, b.states):
+            _to_cons(make_ctx(*e), st)
+        W.check(W.lib().ws_batch_rk3_run(b.handle, nsteps, 0.5, graph, W.stream_handle()))
+        b.sync_states()
+        _sync()
+        runs.append([st.interior("f").cpu().numpy() for st in b.states])
+        b.close()
+    for a, c in zip(*runs):
+        assert np.array_equal(a, c)
+
+
+def test_batch_autotune_keeps_results(ws):
+    extents = _extents(19, 5, 20, 60)
+    b, _ = _batch(extents, 700)
+    s = W.stream_handle()
+    W.check(W.lib().ws_batch_du_comput(b.handle, s))
+    _sync()
+    before = [_du(st) for st in b.states]
+    times = (ctypes.c_float * len(W.TILE_SHAPES))()
+    W.check(W.lib().ws_batch_autotune(b.handle, 2, times, s))
+    chosen = ctypes.c_int32()
+    W.check(W.lib().ws_batch_get_tile_shape(b.handle, ctypes.byref(chosen)))
+    log(f"  batch autotune ms {list(times)} -> {chosen.value}")
+    assert times[chosen.value] == min(times)
+    W.check(W.lib().ws_batch_du_comput(b.handle, s))
+    _sync()
+    for a, st in zip(before, b.states):
+        assert rel_l2(_du(st), a) <= 1e-13

@@ -9,7 +9,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwindstencil_amd.so")
+# the in-tree build; WINDSTENCIL_AMD_LIB names another build of the same library
+LIB_PATH = os.environ.get("WINDSTENCIL_AMD_LIB") or os.path.join(HERE, "libwindstencil_amd.so")
 LAP = 4
 
 WS_OK = 0
@@ -93,6 +94,21 @@ _SIGS = {
     "ws_rk3_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_void_p]),
     "ws_rk3_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.c_double, _i32,
                                   ctypes.c_void_p]),
+    "ws_batch_create": (ctypes.c_int, [ctypes.POINTER(WsConfig), _i32, ctypes.POINTER(_i32), ctypes.POINTER(WsState),
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "ws_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "ws_batch_block": (ctypes.c_int, [ctypes.c_void_p, _i32, ctypes.POINTER(WsState)]),
+    "ws_batch_cells": (ctypes.c_int64, [ctypes.c_void_p]),
+    "ws_batch_set_precision": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_batch_set_kernel": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_batch_set_tile_shape": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_batch_get_tile_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_batch_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ws_batch_du_comput": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ws_batch_cfl_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    "ws_batch_rk3_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "ws_batch_rk3_run": (ctypes.c_int, [ctypes.c_void_p, _i32, ctypes.c_double, _i32, ctypes.c_void_p]),
+    "ws_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, _i32, ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
     # reference-named surface (include/windstencil_amd_ref.h)
     "ws_ref_bind": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.POINTER(WsMetricFields),
                                    ctypes.c_void_p]),
@@ -257,7 +273,45 @@ class TorchState:
         return a[LAP:LAP + nz, LAP:LAP + ny, LAP:LAP + nx]
 
     def set_prims(self, rho, u, v, w, T):
-        """Upload interior primitives (numpy (nz, ny, nx)) and derive f = pri_to_cons."""
+        """Upload interior primitives (numpy (nz, ny, nx)); the conserved state f is
+        left alone (ws_pri_to_cons derives it)."""
         import torch
         for k, a in zip(("d", "u", "v", "w", "T"), (rho, u, v, w, T)):
             self.interior(k).copy_(torch.from_numpy(a))
+
+
+class Batch:
+    """A ws_batch over torch-backed block states (one TorchState per block)."""
+
+    def __init__(self, base_cfg, extents, states=None):
+        self.extents = [tuple(int(v) for v in e) for e in extents]
+        self.states = states if states is not None else [TorchState(*e) for e in self.extents]
+        n = len(self.extents)
+        ext = (_i32 * (3 * n))(*[v for e in self.extents for v in e])
+        arr = (WsState * n)(*[s.ws for s in self.states])
+        h = ctypes.c_void_p()
+        check(lib().ws_batch_create(ctypes.byref(base_cfg), n, ext, arr, ctypes.byref(h)), "ws_batch_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().ws_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cells(self):
+        return int(lib().ws_batch_cells(self.handle))
+
+    def sync_states(self):
+        """Pick up the f/fn exchange of RK3 steps in every block's TorchState."""
+        for i, s in enumerate(self.states):
+            st = WsState()
+            check(lib().ws_batch_block(self.handle, i, ctypes.byref(st)), "ws_batch_block")
+            s.ws.f, s.ws.fn = st.f, st.fn
+            s.sync_from_ws()
+

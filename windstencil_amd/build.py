@@ -1,6 +1,6 @@
 """Build the HIP library in place: windstencil_amd/libwindstencil_amd.so (gfx950).
 
-    python -m windstencil_amd.build [--verbose]
+    python -m windstencil_amd.build [--verbose] [--src DIR] [--out LIB]
 
 Every source under csrc/ is compiled with hipcc for gfx950 only and linked into
 one shared library.  Kernel resource usage (VGPRs, SGPRs, LDS, scratch) is
@@ -28,12 +28,12 @@ COMMON = [
 ]
 
 
-def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+def sources(csrc=CSRC):
+    return sorted(os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith((".hip", ".cpp")))
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+def _compile(src, build_dir=BUILD):
+    obj = os.path.join(build_dir, os.path.basename(src) + ".o")
     cmd = [HIPCC] + COMMON + ["-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", src, "-o", obj]
@@ -43,34 +43,39 @@ def _compile(src):
     return obj, p.stderr
 
 
-def build(verbose=False, jobs=None):
-    os.makedirs(BUILD, exist_ok=True)
-    srcs = sources()
+def build(verbose=False, jobs=None, csrc=CSRC, out=LIB):
+    """Compile every source under `csrc` and link them into `out` (by default the
+    in-tree library the package loads)."""
+    build_dir = BUILD if out == LIB else out + ".build"
+    os.makedirs(build_dir, exist_ok=True)
+    srcs = sources(csrc)
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     objs, usage = [], []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, err in ex.map(_compile, srcs):
+        for obj, err in ex.map(lambda s: _compile(s, build_dir), srcs):
             objs.append(obj)
             usage.append(err)
-    with open(os.path.join(BUILD, "resource_usage.txt"), "w") as f:
+    with open(os.path.join(build_dir, "resource_usage.txt"), "w") as f:
         f.write("\n".join(u for u in usage if u))
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB} from {len(objs)} sources")
-    return LIB
+        print(f"built {out} from {len(objs)} sources")
+    return out
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--src", default=CSRC, help="source directory (default: the package's csrc/)")
+    ap.add_argument("--out", default=LIB, help="library to write (default: the in-tree library)")
     a = ap.parse_args()
     try:
-        build(verbose=True)
+        build(verbose=True, csrc=os.path.abspath(a.src), out=os.path.abspath(a.out))
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -1,0 +1,339 @@
+// batch.cpp -- many periodic blocks of uneven size advanced together.
+//
+// Each block keeps its own context (extents, spacing 1/(n-1)) and its caller-owned
+// state.  The inviscid RHS of all blocks runs as one persistent launch per direction
+// over a device tile list (block, tile coordinates) built here for every tile shape;
+// the ghost fill, sound speed, stage update and primitive recovery run per block.  A
+// batch step uses one dt for every block: the minimum of the blocks' CFL steps.  A
+// whole SSP-RK3 step (CFL reduction included) can be captured and replayed as one
+// hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ws_internal.h"
+#include "ws_tiles.h"
+
+using namespace ws;
+
+struct ws_batch {
+    ws_config base;
+    int precision = WS_PREC_FP64;
+    int kernel = WS_KERNEL_TILED;
+    int tile_shape = WS_TILE_30x16_K2;
+    std::vector<ws_ctx *> ctx;
+    std::vector<ws_state> st;
+    BatchBlock *d_blocks = nullptr;
+    int4 *d_tiles[WS_TILE_COUNT][3] = {};
+    int ntiles[WS_TILE_COUNT][3] = {};
+    double *d_dts = nullptr;    // per-block CFL steps
+    double *d_dt = nullptr;     // the batch step (their minimum)
+    hipStream_t capture_stream = nullptr;
+    long long cells = 0;
+};
+
+namespace {
+
+int fail(ws_batch *b, int r)
+{
+    ws_batch_destroy(b);
+    return r;
+}
+
+// tiles of one block for one direction and shape, in launch order (bx fastest)
+void block_tiles(int b, const int n[3], int shape, int dir, std::vector<int4> &out)
+{
+    const TileShapeDef &D = kTileShapes[shape];
+    const int L = dir == 0 ? D.L0 : D.L1, W = dir == 0 ? D.W0 : D.W1;
+    const int xa = tile_xalign(dir, W);
+    // the box starts at LAP; aligned y/z tiles start at the aligned column below it
+    const int nxa = n[0] + LAP - LAP / xa * xa;
+    int gx, gy, gz;
+    if (dir == 0) { gx = (n[0] + L - 1) / L; gy = (n[1] + W - 1) / W; gz = n[2]; }
+    else if (dir == 1) { gx = (nxa + W - 1) / W; gy = (n[1] + L - 1) / L; gz = n[2]; }
+    else { gx = (nxa + W - 1) / W; gy = (n[2] + L - 1) / L; gz = n[1]; }
+    for (int z = 0; z < gz; ++z)
+        for (int y = 0; y < gy; ++y)
+            for (int x = 0; x < gx; ++x) out.push_back(make_int4(b, x, y, z));
+}
+
+BatchTiles tiles_of(const ws_batch *b)
+{
+    BatchTiles t;
+    t.shape = b->tile_shape;
+    t.blocks = b->d_blocks;
+    for (int d = 0; d < 3; ++d) {
+        t.tiles[d] = b->d_tiles[b->tile_shape][d];
+        t.ntiles[d] = b->ntiles[b->tile_shape][d];
+    }
+    return t;
+}
+
+int batch_rhs(ws_batch *b, hipStream_t s)
+{
+    if (b->kernel == WS_KERNEL_ANCHOR) {
+        // the reference-order kernels have no batched form: one RHS per block
+        for (size_t i = 0; i < b->ctx.size(); ++i) {
+            int r = rhs_for_ctx(b->ctx[i], &b->st[i], s);
+            if (r) return r;
+        }
+        return WS_OK;
+    }
+    const KParams &k = b->ctx[0]->kp;   // physics constants and metrics are the batch's
+    const BatchTiles t = tiles_of(b);
+    return b->precision == WS_PREC_MIXED ? launch_rhs_batch_mixed(k, t, s) : launch_rhs_batch_tiled(k, t, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *extents, const ws_state *states,
+                    ws_batch **out)
+{
+    if (!base || !extents || !states || !out || nblocks <= 0)
+        return set_error(WS_ERR_INVALID, "bad batch request");
+    *out = nullptr;
+    ws_batch *b = new (std::nothrow) ws_batch;
+    if (!b) return set_error(WS_ERR_NOMEM, "batch allocation failed");
+    b->base = *base;
+    b->precision = base->precision;
+    b->kernel = base->kernel;
+    std::vector<BatchBlock> host(nblocks);
+    for (int i = 0; i < nblocks; ++i) {
+        ws_config cfg = *base;
+        cfg.nx = extents[3 * i]; cfg.ny = extents[3 * i + 1]; cfg.nz = extents[3 * i + 2];
+        if (cfg.nx < 2 || cfg.ny < 2 || cfg.nz < 2) return fail(b, set_error(WS_ERR_INVALID, "block extent below 2"));
+        cfg.hx = 1.0 / (cfg.nx - 1); cfg.hy = 1.0 / (cfg.ny - 1); cfg.hz = 1.0 / (cfg.nz - 1);
+        ws_ctx *c = nullptr;
+        int r = ws_ctx_create(&cfg, &c);
+        if (r) return fail(b, r);
+        b->ctx.push_back(c);
+        if ((r = check_state(c, &states[i]))) return fail(b, r);
+        b->st.push_back(states[i]);
+        BatchBlock &h = host[i];
+        std::memset(&h, 0, sizeof(h));
+        h.d = states[i].d; h.u = states[i].u; h.v = states[i].v; h.w = states[i].w; h.cc = states[i].cc;
+        h.du = states[i].du;
+        h.n[0] = cfg.nx; h.n[1] = cfg.ny; h.n[2] = cfg.nz;
+        h.rh[0] = 1.0 / cfg.hx; h.rh[1] = 1.0 / cfg.hy; h.rh[2] = 1.0 / cfg.hz;
+        b->cells += (long long)cfg.nx * cfg.ny * cfg.nz;
+    }
+    hipError_t e = hipMalloc(&b->d_blocks, sizeof(BatchBlock) * nblocks);
+    if (e != hipSuccess) return fail(b, set_error(WS_ERR_NOMEM, "batch block table"));
+    e = hipMemcpy(b->d_blocks, host.data(), sizeof(BatchBlock) * nblocks, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(b, hip_error(e, "batch block table upload"));
+    for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
+        for (int dir = 0; dir < 3; ++dir) {
+            std::vector<int4> list;
+            for (int i = 0; i < nblocks; ++i) block_tiles(i, host[i].n, shape, dir, list);
+            if (list.size() > (size_t)0x7fffffff) return fail(b, set_error(WS_ERR_INVALID, "too many tiles"));
+            b->ntiles[shape][dir] = (int)list.size();
+            e = hipMalloc(&b->d_tiles[shape][dir], sizeof(int4) * list.size());
+            if (e != hipSuccess) return fail(b, set_error(WS_ERR_NOMEM, "batch tile list"));
+            e = hipMemcpy(b->d_tiles[shape][dir], list.data(), sizeof(int4) * list.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(b, hip_error(e, "batch tile list upload"));
+        }
+    e = hipMalloc(&b->d_dts, sizeof(double) * nblocks);
+    if (e == hipSuccess) e = hipMalloc(&b->d_dt, sizeof(double));
+    if (e != hipSuccess) return fail(b, set_error(WS_ERR_NOMEM, "batch dt buffers"));
+    e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(b, hip_error(e, "batch capture stream"));
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(b, hip_error(e, "batch setup"));
+    *out = b;
+    return WS_OK;
+}
+
+int ws_batch_destroy(ws_batch *b)
+{
+    if (!b) return WS_OK;
+    hipDeviceSynchronize();
+    for (ws_ctx *c : b->ctx) ws_ctx_destroy(c);
+    if (b->d_blocks) hipFree(b->d_blocks);
+    for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
+        for (int dir = 0; dir < 3; ++dir)
+            if (b->d_tiles[shape][dir]) hipFree(b->d_tiles[shape][dir]);
+    if (b->d_dts) hipFree(b->d_dts);
+    if (b->d_dt) hipFree(b->d_dt);
+    if (b->capture_stream) hipStreamDestroy(b->capture_stream);
+    delete b;
+    return WS_OK;
+}
+
+int ws_batch_block(const ws_batch *b, int32_t i, ws_state *st)
+{
+    if (!b || !st || i < 0 || i >= (int32_t)b->st.size()) return set_error(WS_ERR_INVALID, "bad block index");
+    *st = b->st[i];
+    return WS_OK;
+}
+
+int64_t ws_batch_cells(const ws_batch *b) { return b ? (int64_t)b->cells : 0; }
+
+int ws_batch_set_precision(ws_batch *b, int32_t precision)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    if (precision != WS_PREC_FP64 && precision != WS_PREC_MIXED) return set_error(WS_ERR_INVALID, "unknown precision");
+    b->precision = precision;
+    for (ws_ctx *c : b->ctx) ws_ctx_set_precision(c, precision);
+    return WS_OK;
+}
+
+int ws_batch_set_kernel(ws_batch *b, int32_t kernel)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    if (kernel != WS_KERNEL_TILED && kernel != WS_KERNEL_ANCHOR) return set_error(WS_ERR_INVALID, "unknown kernel");
+    b->kernel = kernel;
+    for (ws_ctx *c : b->ctx) ws_ctx_set_kernel(c, kernel);
+    return WS_OK;
+}
+
+int ws_batch_set_tile_shape(ws_batch *b, int32_t shape)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    if (shape < 0 || shape >= WS_TILE_COUNT) return set_error(WS_ERR_INVALID, "unknown tile shape");
+    b->tile_shape = shape;
+    return WS_OK;
+}
+
+int ws_batch_get_tile_shape(const ws_batch *b, int32_t *shape)
+{
+    if (!b || !shape) return set_error(WS_ERR_INVALID, "null argument");
+    *shape = b->tile_shape;
+    return WS_OK;
+}
+
+int ws_batch_inviscid_rhs(ws_batch *b, hipStream_t s)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    return batch_rhs(b, s);
+}
+
+int ws_batch_du_comput(ws_batch *b, hipStream_t s)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        int r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
+        if (r) return r;
+    }
+    return batch_rhs(b, s);
+}
+
+int ws_batch_cfl_dt(ws_batch *b, double cfl, double *dt_dev, hipStream_t s)
+{
+    if (!b || !dt_dev || !(cfl > 0.0)) return set_error(WS_ERR_INVALID, "bad CFL request");
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        int r = launch_cfl(*b->ctx[i], b->st[i], cfl, b->d_dts + i, s);
+        if (r) return r;
+    }
+    return launch_min_reduce(b->d_dts, (int)b->ctx.size(), dt_dev, s);
+}
+
+int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    for (ws_state &st : b->st) std::swap(st.f, st.fn);
+    const ws_soa nolap{};
+    for (int krk = 1; krk <= 3; ++krk) {
+        int r = ws_batch_du_comput(b, s);
+        if (r) return r;
+        for (size_t i = 0; i < b->ctx.size(); ++i) {
+            ws_ctx *c = b->ctx[i];
+            ws_state &st = b->st[i];
+            if ((r = launch_time_advance(*c, krk, st.f, st.fn, st.du, nolap, c->cfg.dt, dt_dev, s))) return r;
+            if ((r = launch_cons_to_pri(*c, st.f, st.d, st.u, st.v, st.w, st.T, st.P, s))) return r;
+            if (c->cfg.bc_planes && (r = ws_bc_planes(c, &st, s))) return r;
+        }
+    }
+    return WS_OK;
+}
+
+int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t s)
+{
+    if (!b || nsteps < 0) return set_error(WS_ERR_INVALID, "bad RK3 request");
+    double *dt_dev = cfl > 0.0 ? b->d_dt : nullptr;
+    int r = WS_OK;
+    if (!use_graph) {
+        for (int n = 0; n < nsteps && !r; ++n) {
+            if (dt_dev) r = ws_batch_cfl_dt(b, cfl, dt_dev, s);
+            if (!r) r = ws_batch_rk3_step(b, dt_dev, s);
+        }
+    } else {
+        // one captured graph per parity of the f/fn swap (a step ends with them exchanged);
+        // step 0 runs eagerly
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
+        for (int n = 0; n < nsteps && !r; ++n) {
+            if (n == 0) {
+                if (dt_dev) r = ws_batch_cfl_dt(b, cfl, dt_dev, s);
+                if (!r) r = ws_batch_rk3_step(b, dt_dev, s);
+                continue;
+            }
+            const int p = n & 1;
+            if (!exec[p]) {
+                const std::vector<ws_state> saved = b->st;
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
+                if (e != hipSuccess) { r = hip_error(e, "begin capture"); break; }
+                if (dt_dev) r = ws_batch_cfl_dt(b, cfl, dt_dev, b->capture_stream);
+                if (!r) r = ws_batch_rk3_step(b, dt_dev, b->capture_stream);
+                e = hipStreamEndCapture(b->capture_stream, &g);
+                b->st = saved;   // capture recorded the step; the swap happens at replay
+                if (r) { if (g) hipGraphDestroy(g); break; }
+                if (e != hipSuccess) { r = hip_error(e, "end capture"); break; }
+                e = hipGraphInstantiate(&exec[p], g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (e != hipSuccess) { r = hip_error(e, "graph instantiate"); break; }
+            }
+            hipError_t e = hipGraphLaunch(exec[p], s);
+            if (e != hipSuccess) { r = hip_error(e, "graph launch"); break; }
+            for (ws_state &st : b->st) std::swap(st.f, st.fn);
+        }
+        for (int p = 0; p < 2; ++p)
+            if (exec[p]) {
+                hipStreamSynchronize(s);
+                hipGraphExecDestroy(exec[p]);
+            }
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (!r && e != hipSuccess) r = hip_error(e, "batch RK3 run");
+    return r;
+}
+
+int ws_batch_autotune(ws_batch *b, int32_t reps, float *times_ms, hipStream_t s)
+{
+    if (!b || reps <= 0) return set_error(WS_ERR_INVALID, "bad autotune request");
+    if (b->kernel == WS_KERNEL_ANCHOR) return set_error(WS_ERR_INVALID, "autotune applies to the tiled kernels");
+    hipEvent_t e0, e1;
+    hipError_t e = hipEventCreate(&e0);
+    if (e != hipSuccess) return hip_error(e, "autotune event");
+    e = hipEventCreate(&e1);
+    if (e != hipSuccess) { hipEventDestroy(e0); return hip_error(e, "autotune event"); }
+    const int saved = b->tile_shape;
+    int best = saved, r = WS_OK;
+    float best_ms = 1e30f;
+    for (int shape = 0; shape < WS_TILE_COUNT && !r; ++shape) {
+        b->tile_shape = shape;
+        for (int i = 0; i < 2 && !r; ++i) r = batch_rhs(b, s);
+        if (r) break;
+        hipEventRecord(e0, s);
+        for (int i = 0; i < reps && !r; ++i) r = batch_rhs(b, s);
+        hipEventRecord(e1, s);
+        if (r) break;
+        e = hipEventSynchronize(e1);
+        if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (times_ms) times_ms[shape] = ms;
+        if (ms < best_ms) { best_ms = ms; best = shape; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    b->tile_shape = r ? saved : best;
+    return r;
+}
+
+}  // extern "C"

@@ -174,6 +174,35 @@ int rhs_dispatch(ws_ctx *c, ws_field d, ws_field u, ws_field v, ws_field w, ws_f
 
 }  // namespace
 
+namespace ws {
+
+int check_state(const ws_ctx *c, const ws_state *st)
+{
+    if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
+    int r;
+    if ((r = check_soa(c, st->f, "f")) || (r = check_soa(c, st->fn, "fn")) || (r = check_soa(c, st->du, "du")) ||
+        (r = check_field(c, st->d, "d")) || (r = check_field(c, st->u, "u")) || (r = check_field(c, st->v, "v")) ||
+        (r = check_field(c, st->w, "w")) || (r = check_field(c, st->T, "T")) || (r = check_field(c, st->cc, "cc")))
+        return r;
+    if (st->P.ptr && (r = check_field(c, st->P, "P"))) return r;
+    return WS_OK;
+}
+
+int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s)
+{
+    return rhs_dispatch(c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, interior_job(c), 0, s);
+}
+
+int prepare_rhs_inputs(ws_ctx *c, const ws_state *st, hipStream_t s)
+{
+    const ws_field g[5] = {st->d, st->u, st->v, st->w, st->T};
+    int r = launch_ghost_fill(*c, g, 5, s);
+    if (!r) r = launch_sound_speed(*c, st->T, st->cc, ghosted_job(c), s);
+    return r;
+}
+
+}  // namespace ws
+
 extern "C" {
 
 const char *ws_version(void) { return "windstencil_amd 0.1 (gfx950)"; }

@@ -204,6 +204,20 @@ __global__ void __launch_bounds__(NT) k_cfl_final(const double *partial, int n, 
     if (threadIdx.x == 0) *dt_out = cfl * hmin / red[0];
 }
 
+__global__ void __launch_bounds__(NT) k_min_reduce(const double *v, int n, double *out)
+{
+    __shared__ double red[NT];
+    double mn = v[0];
+    for (int i = threadIdx.x; i < n; i += NT) mn = fmin(mn, v[i]);
+    red[threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
 inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 
 }  // namespace
@@ -286,6 +300,14 @@ int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream
                        dt_dev);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "cfl launch");
+}
+
+int launch_min_reduce(const double *v, int n, double *out, hipStream_t s)
+{
+    if (n <= 0) return set_error(WS_ERR_INVALID, "empty reduction");
+    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(NT), 0, s, v, n, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "min reduce launch");
 }
 
 }  // namespace ws

@@ -11,4 +11,9 @@ int launch_rhs_mixed(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field 
     return launch_rhs_tile<true>(c, d, u, v, w, cc, m, du, job, accumulate, s);
 }
 
+int launch_rhs_batch_mixed(const KParams &k, const BatchTiles &bt, hipStream_t s)
+{
+    return launch_rhs_batch_tile<true>(k, bt, s);
+}
+
 }  // namespace ws

@@ -10,4 +10,9 @@ int launch_rhs_tiled(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field 
     return launch_rhs_tile<false>(c, d, u, v, w, cc, m, du, job, accumulate, s);
 }
 
+int launch_rhs_batch_tiled(const KParams &k, const BatchTiles &bt, hipStream_t s)
+{
+    return launch_rhs_batch_tile<false>(k, bt, s);
+}
+
 }  // namespace ws

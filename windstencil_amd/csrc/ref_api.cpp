@@ -4,7 +4,7 @@
 #include <cstdio>
 #include <cstdlib>
 
-#include "../../include/windstencil_amd_ref.h"
+#include "windstencil_amd_ref.h"
 #include "ws_internal.h"
 
 namespace {

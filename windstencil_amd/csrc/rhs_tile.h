@@ -37,51 +37,60 @@ struct Tile {
     static constexpr int F = L + 1;                        // faces per line
     static constexpr int T = (F + K - 1) / K;              // threads per line
     static constexpr int NT = W * T;                       // threads per block
-    // y/z tiles with 16 columns read one contiguous 128-byte LDS row per quarter-wave
-    // (conflict-free); narrower ones pad the row
-    static constexpr int PS = DIR == 0 ? NL : (W == 16 ? 16 : W + 1);
-    static constexpr int QS = DIR == 0 ? W * NL : NL * PS;   // LDS entries per quantity
+    // every tile stores its lines line-major (point p of line l at l * LS + p), so a
+    // thread's window is contiguous and is read with 128-bit LDS loads.  x tiles fill
+    // their lines along x (LS = NL); y/z tiles fill them across lines (consecutive
+    // threads write the same point of consecutive lines), so their line stride is
+    // padded to NL + 2 (= 2 mod 4 doubles): those writes spread over the banks and
+    // every line still starts 16-byte aligned
+    static constexpr int LS = DIR == 0 ? NL : NL + 2;
+    static constexpr int QS = W * LS;                      // LDS entries per quantity
     static constexpr int NPTS = W * NL;
     static constexpr int ROUNDS = (NPTS + NT - 1) / NT;   // split points per thread (upper bound)
     // y/z tiles of 16 columns start on a multiple of 16 ghosted columns, so that their
     // rows begin on 128-byte lines of the fields (rows of the reference layout are
     // line-aligned when the pitch is a multiple of 16 elements)
-    static constexpr int XALIGN = (DIR != 0 && W == 16) ? 16 : 1;
+    static constexpr int XALIGN = tile_xalign(DIR, W);
     // the last thread of a line may read up to K+6 points past its first face, i.e.
-    // past the line's end; the pad keeps those reads inside the buffer (their values
-    // only reach faces that are never stored)
-    static constexpr int PAD = (K + 1) * (W + 1) + 8;
+    // past the line's end (into the next line, or for the last line past the buffer);
+    // the pad keeps those reads inside the buffer (their values only reach faces that
+    // are never stored)
+    static constexpr int PAD = K + 8;
     static_assert(NT <= 320, "block too large");
     static_assert(5 * W * F <= 10 * QS, "face buffer must fit in the split buffer");
     // LDS index of point p (0..NL-1) of line l (0..W-1)
-    __device__ static int pidx(int l, int p) { return DIR == 0 ? l * NL + p : p * PS + l; }
-    // face buffer index of face f of line l
-    __device__ static int fidx(int l, int f) { return DIR == 0 ? l * F + f : f * W + l; }
+    __device__ static int pidx(int l, int p) { return l * LS + p; }
+    // face buffer index of face f of line l (line-major; F = L + 1 is odd for the even
+    // tile lengths used, so the y/z flux difference, which reads one face of
+    // consecutive lines per lane, touches distinct banks)
+    __device__ static int fidx(int l, int f) { return l * F + f; }
 };
 
+// One tile (tile coordinates bx, by, bz within the box) of one direction, with the
+// block's LDS buffer sp (10 * QS + PAD doubles).  Every thread of the block calls it.
 template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX>
-__global__ void __launch_bounds__(320)
-k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_metric_fields mf,
-           ws_soa du, Box box)
+__device__ __forceinline__ void rhs_tile_body(const KParams &k, const ws_field &d, const ws_field &u,
+                                              const ws_field &v, const ws_field &w, const ws_field &cc,
+                                              const ws_metric_fields &mf, const ws_soa &du, const Box &box,
+                                              int bx, int by, int bz, double *sp)
 {
     using TL = Tile<DIR, L, W, K>;
-    __shared__ double sp[10 * TL::QS + TL::PAD];
     const int t = threadIdx.x;
 
     // tile origin: x0 always tiles x; the line axis is x (DIR 0), y (DIR 1) or z (DIR 2)
     int x0, y0, z0;
     if (DIR == 0) {
-        x0 = box.s[0] + blockIdx.x * L;
-        y0 = box.s[1] + blockIdx.y * W;
-        z0 = box.s[2] + blockIdx.z;
+        x0 = box.s[0] + bx * L;
+        y0 = box.s[1] + by * W;
+        z0 = box.s[2] + bz;
     } else if (DIR == 1) {
-        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + blockIdx.x * W;
-        y0 = box.s[1] + blockIdx.y * L;
-        z0 = box.s[2] + blockIdx.z;
+        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + bx * W;
+        y0 = box.s[1] + by * L;
+        z0 = box.s[2] + bz;
     } else {
-        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + blockIdx.x * W;
-        z0 = box.s[2] + blockIdx.y * L;
-        y0 = box.s[1] + blockIdx.z;
+        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + bx * W;
+        z0 = box.s[2] + by * L;
+        y0 = box.s[1] + bz;
     }
     // cells of the tile that lie in the box, along the line and across lines
     // (lines lo_across .. lim_across-1; only the first aligned y/z tile has lo_across > 0)
@@ -178,15 +187,12 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
     }
 
     // ---- phase 2: K faces of one line per thread, all five variables --------------------
-    int line, seg;
-    if (DIR == 0) { line = t / TL::T; seg = t % TL::T; }
-    else          { line = t % W;     seg = t / W; }
+    const int line = t / TL::T, seg = t % TL::T;
     const int f0 = seg * K;                               // first face of this thread
     const bool active = (t < TL::NT) && (line >= lo_across) && (line < lim_across) && (f0 <= lim_line);
     double Fv[5][K];
     if (active) {
         const int base = TL::pidx(line, f0);
-        const int step = DIR == 0 ? 1 : TL::PS;
 #pragma unroll 1
         for (int m = 0; m < 5; ++m) {
             const double *qp = sp + m * TL::QS + base;
@@ -197,14 +203,14 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
                 // FP32 reconstruction relative to FP64 base points; the merge is FP64
                 double bP[K], bM[K];
                 float cP[K], cM[K];
-                weno_k_mixed<false, K>([&](int j) { return qp[j * step]; }, bP, cP);
-                weno_k_mixed<true, K>([&](int j) { return qm[j * step]; }, bM, cM);
+                weno_k_mixed<false, K>([&](int j) { return qp[j]; }, bP, cP);
+                weno_k_mixed<true, K>([&](int j) { return qm[j]; }, bM, cM);
 #pragma unroll
                 for (int j = 0; j < K; ++j) Fv[m][j] = (bP[j] + bM[j]) + (double)(cP[j] + cM[j]);
             } else {
                 double nP[K], dP[K], nM[K], dM[K];
-                weno_k<false, K>([&](int j) { return qp[j * step]; }, nP, dP);
-                weno_k<true, K>([&](int j) { return qm[j * step]; }, nM, dM);
+                weno_k<false, K>([&](int j) { return qp[j]; }, nP, dP);
+                weno_k<true, K>([&](int j) { return qm[j]; }, nM, dM);
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     const double den = dP[j] * dM[j];
@@ -254,6 +260,47 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
 }
 
 template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX>
+__global__ void __launch_bounds__(320)
+k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_metric_fields mf,
+           ws_soa du, Box box)
+{
+    using TL = Tile<DIR, L, W, K>;
+    __shared__ double sp[10 * TL::QS + TL::PAD];
+    rhs_tile_body<DIR, L, W, K, FIELDS, ACC, MIX>(k, d, u, v, w, cc, mf, du, box, blockIdx.x, blockIdx.y,
+                                                  blockIdx.z, sp);
+}
+
+// Batched form: the tiles of one direction over every block of a batch, in one
+// persistent launch.  Work-group g walks the contiguous range [g*chunk, (g+1)*chunk)
+// of the tile list, so neighbouring tiles (which share halo lines) run on the same
+// work-group, hence the same XCD and L2.  Each block brings its own fields, extents
+// and spacing; the physics constants and metrics are the batch's.
+template <int DIR, int L, int W, int K, bool ACC, bool MIX>
+__global__ void __launch_bounds__(320)
+k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, int chunk)
+{
+    using TL = Tile<DIR, L, W, K>;
+    __shared__ double sp[10 * TL::QS + TL::PAD];
+    const int first = blockIdx.x * chunk;
+    const int last = min(first + chunk, ntiles);
+    const ws_metric_fields nomf{};
+    for (int i = first; i < last; ++i) {
+        const int4 tl = tiles[i];
+        const BatchBlock &b = blocks[tl.x];
+        KParams kb = k;
+        kb.nx = b.n[0]; kb.ny = b.n[1]; kb.nz = b.n[2];
+        kb.nyg = b.n[1] + 2 * LAP; kb.nzg = b.n[2] + 2 * LAP;
+        kb.rh[DIR] = b.rh[DIR];
+        Box box;
+        box.s[0] = box.s[1] = box.s[2] = LAP;
+        box.e[0] = b.n[0] + LAP; box.e[1] = b.n[1] + LAP; box.e[2] = b.n[2] + LAP;
+        rhs_tile_body<DIR, L, W, K, false, ACC, MIX>(kb, b.d, b.u, b.v, b.w, b.cc, nomf, b.du, box, tl.y, tl.z,
+                                                     tl.w, sp);
+        __syncthreads();   // the next tile reuses the LDS buffer
+    }
+}
+
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX>
 void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                 const ws_metric_fields &mf, ws_soa du, const Box &box, hipStream_t s)
 {
@@ -279,22 +326,64 @@ void launch_all(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     launch_dir<2, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, s);
 }
 
+template <int S, bool FIELDS, bool MIX>
+void launch_shape_n(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+                    const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, hipStream_t s)
+{
+    constexpr TileShapeDef D = kTileShapes[S];
+    launch_all<D.L0, D.W0, D.L1, D.W1, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s);
+}
+
 template <bool FIELDS, bool MIX>
 void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                   const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, hipStream_t s)
 {
     switch (shape) {
-    case WS_TILE_15x16_K1:  // x: 15 cells x 16 rows; y, z: 16 cells x 15 columns; one face per thread
-        launch_all<15, 16, 16, 15, 1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s);
-        break;
-    case WS_TILE_30x16_K2:  // x: 30 cells x 16 rows; y, z: 32 cells x 15 columns; two faces per thread
-        launch_all<30, 16, 32, 15, 2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s);
-        break;
-    case WS_TILE_30x16_A16_K2:  // as above, y/z tiles 16 aligned columns (272 threads)
-    default:
-        launch_all<30, 16, 32, 16, 2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s);
-        break;
+    case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s); break;
+    case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s); break;
+    default: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s); break;
     }
+}
+
+// ---- batched launches ------------------------------------------------------------------
+
+// work-groups of k_rhs_batch that fit on the device at once (the persistent grid)
+template <int DIR, int L, int W, int K, bool ACC, bool MIX>
+int resident_groups()
+{
+    static const int n = [] {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<DIR, L, W, K, ACC, MIX>,
+                                                         Tile<DIR, L, W, K>::NT, 0) != hipSuccess || per <= 0)
+            per = 1;
+        return cus * per;
+    }();
+    return n;
+}
+
+template <int DIR, int L, int W, int K, bool ACC, bool MIX>
+void launch_batch_dir(const KParams &k, const BatchTiles &bt, hipStream_t s)
+{
+    const int n = bt.ntiles[DIR];
+    if (n <= 0) return;
+    int grid = resident_groups<DIR, L, W, K, ACC, MIX>();
+    if (grid > n) grid = n;
+    const int chunk = (n + grid - 1) / grid;
+    grid = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL((k_rhs_batch<DIR, L, W, K, ACC, MIX>), dim3(grid), dim3(Tile<DIR, L, W, K>::NT), 0, s, k,
+                       bt.blocks, bt.tiles[DIR], n, chunk);
+}
+
+template <int S, bool MIX>
+void launch_batch_shape_n(const KParams &k, const BatchTiles &bt, hipStream_t s)
+{
+    constexpr TileShapeDef D = kTileShapes[S];
+    launch_batch_dir<0, D.L0, D.W0, D.K, false, MIX>(k, bt, s);
+    launch_batch_dir<1, D.L1, D.W1, D.K, true, MIX>(k, bt, s);
+    launch_batch_dir<2, D.L1, D.W1, D.K, true, MIX>(k, bt, s);
 }
 
 template <bool MIX>
@@ -314,6 +403,19 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, MIX ? "mixed-precision RHS launch" : "tiled RHS launch");
+    return WS_OK;
+}
+
+template <bool MIX>
+int launch_rhs_batch_tile(const KParams &k, const BatchTiles &bt, hipStream_t s)
+{
+    switch (bt.shape) {
+    case 0: launch_batch_shape_n<0, MIX>(k, bt, s); break;
+    case 1: launch_batch_shape_n<1, MIX>(k, bt, s); break;
+    default: launch_batch_shape_n<2, MIX>(k, bt, s); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, MIX ? "batched mixed-precision RHS launch" : "batched RHS launch");
     return WS_OK;
 }
 

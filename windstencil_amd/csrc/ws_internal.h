@@ -5,7 +5,7 @@
 #include <cstdint>
 #include <string>
 
-#include "../../include/windstencil_amd.h"
+#include "windstencil_amd.h"
 
 namespace ws {
 
@@ -48,6 +48,24 @@ struct Ctx {
 
 KParams make_kparams(const ws_config &cfg);
 
+// One block of a batch, as the batched RHS kernels see it.
+struct BatchBlock {
+    ws_field d, u, v, w, cc;
+    ws_soa du;
+    int n[3];            // interior extents
+    int pad;
+    double rh[3];        // 1/h per direction
+};
+
+// Device tile lists of a batch for one tile shape: entry (block, bx, by, bz) per tile,
+// one list per direction.
+struct BatchTiles {
+    int shape;
+    const BatchBlock *blocks;
+    const int4 *tiles[3];
+    int ntiles[3];
+};
+
 // tile shape the autotuner found best for these extents and precision, or -1
 int remembered_shape(int nx, int ny, int nz, int precision);
 
@@ -63,6 +81,16 @@ int launch_rhs_tiled(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field 
 int launch_rhs_mixed(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
 
+int launch_rhs_batch_tiled(const KParams &k, const BatchTiles &bt, hipStream_t s);
+int launch_rhs_batch_mixed(const KParams &k, const BatchTiles &bt, hipStream_t s);
+
+// library-internal helpers of the C API (capi.cpp)
+int check_state(const ws_ctx *c, const ws_state *st);
+// ghost fill of d, u, v, w, T and sound speed over the ghosted box
+int prepare_rhs_inputs(ws_ctx *c, const ws_state *st, hipStream_t s);
+// the RHS over the interior with the context's kernel and precision (overwrites du)
+int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s);
+
 int launch_sound_speed(const Ctx &c, ws_field T, ws_field cc, ws_job job, hipStream_t s);
 int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s);
 int launch_cons_to_pri(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w,
@@ -74,6 +102,8 @@ int launch_time_advance(const Ctx &c, int krk, ws_soa f, ws_soa fn, ws_soa du, w
 int launch_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy, uint32_t sz,
                 hipStream_t s);
 int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream_t s);
+// *out = min(v[0..n-1]) (one work-group, no atomics)
+int launch_min_reduce(const double *v, int n, double *out, hipStream_t s);
 
 }  // namespace ws
 
