@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-second", action="store_true",
                     help="skip timing the other precision (mixed when --precision fp64)")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape (default: autotune)")
+    ap.add_argument("--schedule", type=int, default=-1,
+                    help="with --tile: force a schedule (0 grid, 1 persistent, 2 persistent + prefetch)")
     ap.add_argument("--workload", default="rhs", choices=["rhs", "batched", "rk3"],
                     help="rhs: the headline RHS line (default); batched: 48 uneven blocks; "
                          "rk3: SSP-RK3 steps of one box in a graph")
@@ -142,12 +144,17 @@ def bench_batched(args):
     tune = None
     if args.tile >= 0:
         W.check(L.ws_batch_set_tile_shape(b.handle, args.tile))
+        if args.schedule >= 0:
+            W.check(L.ws_batch_set_schedule(b.handle, args.schedule))
     else:
-        times = (ctypes.c_float * len(W.TILE_SHAPES))()
+        ns = len(W.SCHEDULES)
+        times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
         W.check(L.ws_batch_autotune(b.handle, 3, times, sh))
-        tune = {str(s): round(times[s], 4) for s in W.TILE_SHAPES}
+        tune = {f"{s}/{q}": round(times[s * ns + q], 4) for s in W.TILE_SHAPES for q in W.SCHEDULES}
     chosen = ctypes.c_int32()
     W.check(L.ws_batch_get_tile_shape(b.handle, ctypes.byref(chosen)))
+    sched = ctypes.c_int32()
+    W.check(L.ws_batch_get_schedule(b.handle, ctypes.byref(sched)))
     for _ in range(args.warmup):
         W.check(L.ws_batch_du_comput(b.handle, sh))
     torch.cuda.synchronize()
@@ -179,7 +186,8 @@ def bench_batched(args):
                             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
                             "kernel": "k_rhs_batch x3 directions (one RHS of every block)",
                             "kernel_ms": round(kern_ms, 4)},
-                  tile_shape=chosen.value, autotune_ms=tune,
+                  tile_shape=chosen.value, schedule=sched.value, autotune_ms=tune,
+                  autotune_key="tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
                   rk3={"steps": nsteps, "graph": True, "seconds": round(rk3_s, 4),
                        "mcell_updates_per_s": round(N * 3 * nsteps / rk3_s / 1e6, 2),
                        "note": "N x 3 stages x steps / wall time; CFL 0.5 from the per-block minimum each step"})
@@ -292,12 +300,16 @@ def main():
         tune = None
         if args.tile >= 0:
             W.check(L.ws_ctx_set_tile_shape(ctx.handle, args.tile))
+            if args.schedule >= 0:
+                W.check(L.ws_ctx_set_schedule(ctx.handle, args.schedule))
         else:
-            times = (ctypes.c_float * len(W.TILE_SHAPES))()
+            ns = len(W.SCHEDULES)
+            times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
             W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 5, times, sh))
-            tune = {str(s): round(times[s], 4) for s in W.TILE_SHAPES}
-        chosen = ctypes.c_int32()
+            tune = {f"{s}/{q}": round(times[s * ns + q], 4) for s in W.TILE_SHAPES for q in W.SCHEDULES}
+        chosen, sched = ctypes.c_int32(), ctypes.c_int32()
         W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
+        W.check(L.ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
         for _ in range(args.warmup):
             rhs()
         torch.cuda.synchronize()
@@ -312,7 +324,7 @@ def main():
         torch.cuda.synchronize()
         prep = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
         kern = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
-        return wall / args.steps * 1e3, prep, kern, chosen.value, tune
+        return wall / args.steps * 1e3, prep, kern, f"{chosen.value}/{sched.value}", tune
 
     other = W.WS_PREC_MIXED if prec == W.WS_PREC_FP64 else W.WS_PREC_FP64
     second = None
@@ -322,7 +334,7 @@ def main():
         du2 = st.interior("du").clone()
         second = {"precision": "mixed" if other == W.WS_PREC_MIXED else "fp64",
                   "value": round(N / (ms2 * 1e-3) / 1e6, 2), "ms_per_step": round(ms2, 4),
-                  "kernel_ms": round(kern2, 4), "tile_shape": shape2, "autotune_ms": tune2}
+                  "kernel_ms": round(kern2, 4), "tile_shape/schedule": shape2, "autotune_ms": tune2}
     ms, prep_ms, kern_ms, shape, tune = run(prec)
     value = N / (ms * 1e-3) / 1e6
     flops_cell, flops_dir = algo_flops()
@@ -375,7 +387,8 @@ def main():
                      "kernel": "k_rhs_tile x3 directions (one RHS)",
                      "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
         "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
-        "tile_shape": shape, "autotune_ms": tune,
+        "tile_shape/schedule": shape, "autotune_ms": tune,
+        "autotune_key": "tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
         "parity_rel_l2_vs_anchor": parity,
         "other_precision": second,
     }

@@ -58,6 +58,13 @@ extern "C" {
 #define WS_TILE_30x16_A16_K2 2   /* x: 30x16, y/z: 32x16 on 16-aligned columns, two faces per thread */
 #define WS_TILE_COUNT 3
 
+/* how the tiled kernels walk their tiles (ws_ctx_set_schedule, ws_batch_set_schedule) */
+#define WS_SCHED_GRID 0        /* one tile per work-group, grid = tile count */
+#define WS_SCHED_PERSISTENT 1  /* resident work-groups, each a contiguous run of tiles */
+#define WS_SCHED_PREFETCH 2    /* as 1, the next tile's loads issued during this tile's
+                                  reconstruction */
+#define WS_SCHED_COUNT 3
+
 /* Ghosted scalar field; layout of cudaField under __HIPCC__ (cuda_commen.h:48-54). */
 typedef struct ws_field {
     double *ptr;
@@ -132,6 +139,10 @@ int ws_ctx_set_precision(ws_ctx *ctx, int32_t precision);
 int ws_ctx_set_kernel(ws_ctx *ctx, int32_t kernel);
 int ws_ctx_set_tile_shape(ws_ctx *ctx, int32_t shape);
 int ws_ctx_get_tile_shape(const ws_ctx *ctx, int32_t *shape);
+/* WS_SCHED_*; the persistent schedules apply without metric fields (with them the
+ * grid schedule runs) */
+int ws_ctx_set_schedule(ws_ctx *ctx, int32_t schedule);
+int ws_ctx_get_schedule(const ws_ctx *ctx, int32_t *schedule);
 
 /* Allocate / free the solver state with the reference's extents
  * (replaces opencfd_mem_init_dev, src/runtime/OCFD_init.cpp:411-446). */
@@ -197,10 +208,10 @@ int ws_mem_value_init(double value, double *ptr, uint32_t pitch, uint32_t size_x
  * double).  No counterpart: the reference uses a fixed DT. */
 int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipStream_t stream);
 
-/* Time every tile shape of the tiled kernels, in the context's precision, on this state
- * (two untimed and `reps` timed RHS launches each, HIP events on `stream`), keep the
- * fastest for the context and remember it for contexts of the same extents and precision.
- * times_ms (WS_TILE_COUNT entries) may be NULL.  Overwrites st->du; synchronises the
+/* Time every (tile shape, schedule) pair of the tiled kernels, in the context's precision,
+ * on this state (two untimed and `reps` timed RHS launches each, HIP events on `stream`),
+ * keep the fastest for the context and remember it for contexts of the same extents and
+ * precision.  times_ms (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major) may be NULL.  Overwrites st->du; synchronises the
  * stream.  No counterpart in the reference. */
 int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
 
@@ -217,6 +228,29 @@ int ws_rk3_step(ws_ctx *ctx, ws_state *st, const double *dt_dev, hipStream_t str
 int ws_rk3_run(ws_ctx *ctx, ws_state *st, int32_t nsteps, double cfl, int32_t use_graph,
                hipStream_t stream);
 
+/* ---- restart files --------------------------------------------------------------
+ * The reference's restart format OCFD%08d.dat (written by OCFD_save, src/io/OCFD_IO.c:153-226
+ * with the per-plane records of write_3d, src/io/OCFD_IO_mpi.c:605-701; read by read_file,
+ * src/io/OCFD_IO.c:24-90, and read_3d, src/io/OCFD_IO_mpi.c:563-604): a Fortran-style record
+ * int32 12 | int32 Istep | float64 tt | int32 12, then rho, u, v, w, T as NZ records each
+ * (one per z plane, x fastest) framed by int32 NX*NY*8 markers; native byte order.  Only the
+ * interior is stored; ghost layers are filled by the next ws_du_comput. */
+
+/* "OCFD%08d.dat" for step into buf. */
+int ws_restart_name(int32_t step, char *buf, int32_t buflen);
+/* Host arrays: fields[0..4] = rho, u, v, w, T, nx*ny*nz each, x fastest. */
+int ws_restart_write_host(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t istep, double tt,
+                          const double *const *fields);
+/* Fails (WS_ERR_INVALID) unless the file holds exactly this grid's records. */
+int ws_restart_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t *istep, double *tt,
+                         double *const *fields);
+/* The interior of st's d, u, v, w, T to / from a restart file (OCFD_save / read_file for
+ * one process).  Synchronise `stream` first; the copies themselves are blocking. */
+int ws_save_restart(ws_ctx *ctx, const ws_state *st, int32_t istep, double tt, const char *path,
+                    hipStream_t stream);
+int ws_load_restart(ws_ctx *ctx, ws_state *st, const char *path, int32_t *istep, double *tt,
+                    hipStream_t stream);
+
 /* ---- batches of blocks ------------------------------------------------------
  * Many independent periodic blocks of uneven size advanced together.  No counterpart
  * in the reference, which runs one block per process (its multi-block runs are MPI
@@ -226,6 +260,7 @@ int ws_rk3_run(ws_ctx *ctx, ws_state *st, int32_t nsteps, double cfl, int32_t us
  * blocks is one persistent launch per direction over a tile list; one dt (the minimum
  * of the blocks' CFL steps) advances every block. */
 typedef struct ws_batch ws_batch;
+
 
 /* extents: 3*nblocks (nx, ny, nz per block); states: nblocks states whose buffers stay
  * owned by the caller and must outlive the batch.  base supplies everything else
@@ -240,6 +275,8 @@ int ws_batch_set_precision(ws_batch *batch, int32_t precision);
 int ws_batch_set_kernel(ws_batch *batch, int32_t kernel);
 int ws_batch_set_tile_shape(ws_batch *batch, int32_t shape);
 int ws_batch_get_tile_shape(const ws_batch *batch, int32_t *shape);
+int ws_batch_set_schedule(ws_batch *batch, int32_t schedule);
+int ws_batch_get_schedule(const ws_batch *batch, int32_t *schedule);
 /* RHS of every block from ghost-filled d,u,v,w and cc (overwrites du). */
 int ws_batch_inviscid_rhs(ws_batch *batch, hipStream_t stream);
 /* ws_du_comput for every block: ghost fill, sound speed, then the batched RHS. */
@@ -251,7 +288,9 @@ int ws_batch_rk3_step(ws_batch *batch, const double *dt_dev, hipStream_t stream)
 /* nsteps batch RK3 steps, dt from the CFL reduction (cfl > 0) or cfg.dt; use_graph = 1
  * captures a whole step (CFL included) in one hipGraph and replays it. */
 int ws_batch_rk3_run(ws_batch *batch, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t stream);
-/* Time every tile shape of the batched RHS and keep the fastest (as ws_autotune). */
+/* Time every (tile shape, schedule) pair of the batched RHS and keep the fastest (as
+ * ws_autotune); times_ms: WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major,
+ * or NULL. */
 int ws_batch_autotune(ws_batch *batch, int32_t reps, float *times_ms, hipStream_t stream);
 
 #ifdef __cplusplus

@@ -43,13 +43,15 @@ def _du(st):
     return st.interior("du").cpu().numpy()
 
 
+@pytest.mark.parametrize("schedule", W.SCHEDULES)
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
 @pytest.mark.parametrize("tile", W.TILE_SHAPES)
-def test_batch_rhs_equals_single_block_runs(ws, tile, precision):
+def test_batch_rhs_equals_single_block_runs(ws, tile, precision, schedule):
     extents = _extents(7, 9, 5, 47)
-    log(f"batch rhs tile {tile} precision {precision}: {extents}")
+    log(f"batch rhs tile {tile} precision {precision} schedule {schedule}: {extents}")
     b, prims = _batch(extents, 100, precision=precision)
     W.check(W.lib().ws_batch_set_tile_shape(b.handle, tile))
+    W.check(W.lib().ws_batch_set_schedule(b.handle, schedule))
     W.check(W.lib().ws_batch_du_comput(b.handle, W.stream_handle()))
     _sync()
     for e, p, st in zip(extents, prims, b.states):
@@ -146,12 +148,14 @@ def test_batch_autotune_keeps_results(ws):
     W.check(W.lib().ws_batch_du_comput(b.handle, s))
     _sync()
     before = [_du(st) for st in b.states]
-    times = (ctypes.c_float * len(W.TILE_SHAPES))()
+    ns = len(W.SCHEDULES)
+    times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
     W.check(W.lib().ws_batch_autotune(b.handle, 2, times, s))
-    chosen = ctypes.c_int32()
-    W.check(W.lib().ws_batch_get_tile_shape(b.handle, ctypes.byref(chosen)))
-    log(f"  batch autotune ms {list(times)} -> {chosen.value}")
-    assert times[chosen.value] == min(times)
+    shape, sched = ctypes.c_int32(), ctypes.c_int32()
+    W.check(W.lib().ws_batch_get_tile_shape(b.handle, ctypes.byref(shape)))
+    W.check(W.lib().ws_batch_get_schedule(b.handle, ctypes.byref(sched)))
+    log(f"  batch autotune ms {list(times)} -> {shape.value}/{sched.value}")
+    assert times[shape.value * ns + sched.value] == min(times)
     W.check(W.lib().ws_batch_du_comput(b.handle, s))
     _sync()
     for a, st in zip(before, b.states):

@@ -66,17 +66,44 @@ def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
     ctx = make_ctx(nx, ny, nz)
     st = upload_state(prims)
     gpu_du_comput(ctx, st)
-    times = (ctypes.c_float * len(W.TILE_SHAPES))()
+    ns = len(W.SCHEDULES)
+    times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
     W.check(W.lib().ws_autotune(ctx.handle, ctypes.byref(st.ws), 3, times, W.stream_handle()))
     log(f"  autotune ms {list(times)}")
-    chosen = ctypes.c_int32()
+    chosen, sched = ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
-    assert times[chosen.value] == min(times)
+    W.check(W.lib().ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
+    assert times[chosen.value * ns + sched.value] == min(times)
     ctx2 = make_ctx(nx, ny, nz)
-    again = ctypes.c_int32()
+    again, again_s = ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_ctx_get_tile_shape(ctx2.handle, ctypes.byref(again)))
-    assert again.value == chosen.value
+    W.check(W.lib().ws_ctx_get_schedule(ctx2.handle, ctypes.byref(again_s)))
+    assert (again.value, again_s.value) == (chosen.value, sched.value)
     assert rel_l2(gpu_du_comput(ctx2, upload_state(prims)), ref) <= TOL_FP64
+
+
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+@pytest.mark.parametrize("tile", W.TILE_SHAPES)
+def test_schedules_agree_bitwise(ws, tile, precision):
+    """The grid, persistent and prefetching schedules run the same per-tile arithmetic:
+    their RHS agree bit for bit, on the full box and on a sub-box job."""
+    for shape in [(31, 17, 19), (67, 45, 37), (130, 40, 33)]:
+        nx, ny, nz = shape
+        prims = random_state(nx, ny, nz, 41, "shock")
+        outs, subs = [], []
+        for sched in W.SCHEDULES:
+            ctx = make_ctx(nx, ny, nz, precision=precision)
+            ctx.set_tile_shape(tile)
+            ctx.set_schedule(sched)
+            outs.append(gpu_du_comput(ctx, upload_state(prims)))
+            st = upload_state(prims)
+            st.t["du"].fill_(3.0)
+            subs.append(gpu_du_comput(ctx, st, job=W.job((4 + 3, 4 + 2, 4 + 1), (4 + nx - 2, 4 + ny - 3, 4 + nz - 1))))
+        log(f"schedules {shape} tile {tile} precision {precision}")
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+        for o in subs[1:]:
+            assert np.array_equal(o, subs[0])
 
 
 def test_tiled_is_independent_of_pitch(ws):

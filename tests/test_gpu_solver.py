@@ -175,3 +175,32 @@ def test_reference_named_entry_points(ws):
     L.OCFD_time_advance(7)
     assert L.ws_ref_last_status() == -1
     L.ws_ref_set_abort_on_error(1)
+
+
+def test_restart_save_and_load_device_state(ws, tmp_path):
+    """ws_save_restart writes the interior of d, u, v, w, T in the reference's format;
+    ws_load_restart puts it back into a state of another pitch, ghosts untouched."""
+    import torch
+    nx, ny, nz = 23, 14, 9
+    L = W.lib()
+    s = W.stream_handle()
+    ctx = make_ctx(nx, ny, nz)
+    prims = random_state(nx, ny, nz, 77, "rough")
+    st = upload_state(prims)
+    path = str(tmp_path / "OCFD00000005.dat").encode()
+    W.check(L.ws_save_restart(ctx.handle, ctypes.byref(st.ws), 5, 0.25, path, s))
+    out = [np.zeros((nz, ny, nx)) for _ in range(5)]
+    istep, tt = ctypes.c_int32(), ctypes.c_double()
+    W.check(L.ws_restart_read_host(path, nx, ny, nz, ctypes.byref(istep), ctypes.byref(tt),
+                                   (ctypes.c_void_p * 5)(*[a.ctypes.data for a in out])))
+    assert istep.value == 5 and tt.value == 0.25
+    for a, p in zip(out, prims):
+        assert np.array_equal(a, p)
+    st2 = W.TorchState(nx, ny, nz, pad=5)
+    W.check(L.ws_load_restart(ctx.handle, ctypes.byref(st2.ws), path, ctypes.byref(istep), ctypes.byref(tt), s))
+    _sync()
+    for k, p in zip(("d", "u", "v", "w", "T"), prims):
+        assert np.array_equal(st2.interior(k).cpu().numpy(), p)
+        full = st2.t[k].clone()
+        full[LAP:LAP + nz, LAP:LAP + ny, LAP:LAP + nx] = 0
+        assert float(full.abs().max()) == 0.0

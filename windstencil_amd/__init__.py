@@ -14,7 +14,10 @@ LIB_PATH = os.environ.get("WINDSTENCIL_AMD_LIB") or os.path.join(HERE, "libwinds
 LAP = 4
 
 WS_OK = 0
+WS_ERR_INVALID, WS_ERR_UNSUPPORTED, WS_ERR_HIP, WS_ERR_NOMEM = -1, -2, -3, -4
 WS_PREC_FP64, WS_PREC_MIXED = 0, 1
+WS_SCHED_GRID, WS_SCHED_PERSISTENT, WS_SCHED_PREFETCH = 0, 1, 2
+SCHEDULES = (WS_SCHED_GRID, WS_SCHED_PERSISTENT, WS_SCHED_PREFETCH)
 WS_KERNEL_TILED, WS_KERNEL_ANCHOR = 0, 1
 WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2 = 0, 1, 2
 TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2)
@@ -72,6 +75,8 @@ _SIGS = {
     "ws_ctx_set_kernel": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_ctx_set_tile_shape": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_ctx_get_tile_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_ctx_set_schedule": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_ctx_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_state_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_state_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p] + [WsField] * 5 +
@@ -94,6 +99,15 @@ _SIGS = {
     "ws_rk3_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_void_p]),
     "ws_rk3_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.c_double, _i32,
                                   ctypes.c_void_p]),
+    "ws_restart_name": (ctypes.c_int, [_i32, ctypes.c_char_p, _i32]),
+    "ws_restart_write_host": (ctypes.c_int, [ctypes.c_char_p, _i32, _i32, _i32, _i32, ctypes.c_double,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "ws_restart_read_host": (ctypes.c_int, [ctypes.c_char_p, _i32, _i32, _i32, ctypes.POINTER(_i32),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_void_p)]),
+    "ws_save_restart": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.c_double,
+                                       ctypes.c_char_p, ctypes.c_void_p]),
+    "ws_load_restart": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_char_p, ctypes.POINTER(_i32),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "ws_batch_create": (ctypes.c_int, [ctypes.POINTER(WsConfig), _i32, ctypes.POINTER(_i32), ctypes.POINTER(WsState),
                                        ctypes.POINTER(ctypes.c_void_p)]),
     "ws_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -103,6 +117,8 @@ _SIGS = {
     "ws_batch_set_kernel": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_batch_set_tile_shape": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_batch_get_tile_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_batch_set_schedule": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_batch_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_batch_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ws_batch_du_comput": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ws_batch_cfl_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
@@ -202,6 +218,9 @@ class Context:
 
     def set_tile_shape(self, s):
         check(lib().ws_ctx_set_tile_shape(self.handle, s), "ws_ctx_set_tile_shape")
+
+    def set_schedule(self, s):
+        check(lib().ws_ctx_set_schedule(self.handle, s), "ws_ctx_set_schedule")
 
 
 # ---- torch-backed buffers ---------------------------------------------------------------------

@@ -24,6 +24,7 @@ struct ws_batch {
     int precision = WS_PREC_FP64;
     int kernel = WS_KERNEL_TILED;
     int tile_shape = WS_TILE_30x16_K2;
+    int schedule = WS_SCHED_PREFETCH;
     std::vector<ws_ctx *> ctx;
     std::vector<ws_state> st;
     BatchBlock *d_blocks = nullptr;
@@ -64,6 +65,7 @@ BatchTiles tiles_of(const ws_batch *b)
 {
     BatchTiles t;
     t.shape = b->tile_shape;
+    t.schedule = b->schedule;
     t.blocks = b->d_blocks;
     for (int d = 0; d < 3; ++d) {
         t.tiles[d] = b->d_tiles[b->tile_shape][d];
@@ -206,6 +208,21 @@ int ws_batch_get_tile_shape(const ws_batch *b, int32_t *shape)
     return WS_OK;
 }
 
+int ws_batch_set_schedule(ws_batch *b, int32_t schedule)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    if (schedule < 0 || schedule >= WS_SCHED_COUNT) return set_error(WS_ERR_INVALID, "unknown schedule");
+    b->schedule = schedule;
+    return WS_OK;
+}
+
+int ws_batch_get_schedule(const ws_batch *b, int32_t *schedule)
+{
+    if (!b || !schedule) return set_error(WS_ERR_INVALID, "null argument");
+    *schedule = b->schedule;
+    return WS_OK;
+}
+
 int ws_batch_inviscid_rhs(ws_batch *b, hipStream_t s)
 {
     if (!b) return set_error(WS_ERR_INVALID, "null batch");
@@ -311,28 +328,31 @@ int ws_batch_autotune(ws_batch *b, int32_t reps, float *times_ms, hipStream_t s)
     if (e != hipSuccess) return hip_error(e, "autotune event");
     e = hipEventCreate(&e1);
     if (e != hipSuccess) { hipEventDestroy(e0); return hip_error(e, "autotune event"); }
-    const int saved = b->tile_shape;
-    int best = saved, r = WS_OK;
+    const int saved_shape = b->tile_shape, saved_sched = b->schedule;
+    int best_shape = saved_shape, best_sched = saved_sched, r = WS_OK;
     float best_ms = 1e30f;
-    for (int shape = 0; shape < WS_TILE_COUNT && !r; ++shape) {
-        b->tile_shape = shape;
-        for (int i = 0; i < 2 && !r; ++i) r = batch_rhs(b, s);
-        if (r) break;
-        hipEventRecord(e0, s);
-        for (int i = 0; i < reps && !r; ++i) r = batch_rhs(b, s);
-        hipEventRecord(e1, s);
-        if (r) break;
-        e = hipEventSynchronize(e1);
-        if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)reps;
-        if (times_ms) times_ms[shape] = ms;
-        if (ms < best_ms) { best_ms = ms; best = shape; }
-    }
+    for (int shape = 0; shape < WS_TILE_COUNT && !r; ++shape)
+        for (int sched = 0; sched < WS_SCHED_COUNT && !r; ++sched) {
+            b->tile_shape = shape;
+            b->schedule = sched;
+            for (int i = 0; i < 2 && !r; ++i) r = batch_rhs(b, s);
+            if (r) break;
+            hipEventRecord(e0, s);
+            for (int i = 0; i < reps && !r; ++i) r = batch_rhs(b, s);
+            hipEventRecord(e1, s);
+            if (r) break;
+            e = hipEventSynchronize(e1);
+            if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            ms /= (float)reps;
+            if (times_ms) times_ms[shape * WS_SCHED_COUNT + sched] = ms;
+            if (ms < best_ms) { best_ms = ms; best_shape = shape; best_sched = sched; }
+        }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    b->tile_shape = r ? saved : best;
+    b->tile_shape = r ? saved_shape : best_shape;
+    b->schedule = r ? saved_sched : best_sched;
     return r;
 }
 

@@ -250,8 +250,11 @@ int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
     if (!c) return set_error(WS_ERR_NOMEM, "host allocation failed");
     c->cfg = *cfg;
     c->kp = make_kparams(*cfg);
-    const int shape = remembered_shape(cfg->nx, cfg->ny, cfg->nz, cfg->precision);
-    if (shape >= 0) c->tile_shape = shape;
+    int shape, sched;
+    if (remembered_choice(cfg->nx, cfg->ny, cfg->nz, cfg->precision, &shape, &sched)) {
+        c->tile_shape = shape;
+        c->schedule = sched;
+    }
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return hip_error(e, "hipGetDevice"); }
     c->cfl_blocks = 1024;
@@ -284,8 +287,11 @@ int ws_ctx_set_precision(ws_ctx *c, int32_t precision)
     if (!c) return set_error(WS_ERR_INVALID, "null context");
     if (precision != WS_PREC_FP64 && precision != WS_PREC_MIXED) return set_error(WS_ERR_INVALID, "unknown precision");
     c->cfg.precision = precision;
-    const int shape = remembered_shape(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision);
-    if (shape >= 0) c->tile_shape = shape;
+    int shape, sched;
+    if (remembered_choice(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision, &shape, &sched)) {
+        c->tile_shape = shape;
+        c->schedule = sched;
+    }
     return WS_OK;
 }
 
@@ -302,6 +308,21 @@ int ws_ctx_set_tile_shape(ws_ctx *c, int32_t shape)
     if (!c) return set_error(WS_ERR_INVALID, "null context");
     if (shape < 0 || shape >= WS_TILE_COUNT) return set_error(WS_ERR_INVALID, "unknown tile shape");
     c->tile_shape = shape;
+    return WS_OK;
+}
+
+int ws_ctx_set_schedule(ws_ctx *c, int32_t schedule)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (schedule < 0 || schedule >= WS_SCHED_COUNT) return set_error(WS_ERR_INVALID, "unknown schedule");
+    c->schedule = schedule;
+    return WS_OK;
+}
+
+int ws_ctx_get_schedule(const ws_ctx *c, int32_t *schedule)
+{
+    if (!c || !schedule) return set_error(WS_ERR_INVALID, "null argument");
+    *schedule = c->schedule;
     return WS_OK;
 }
 

@@ -1,0 +1,171 @@
+// restart.cpp -- restart files in the reference's format (OCFD%08d.dat).
+//
+// Layout (src/io/OCFD_IO.c:153-226 writes it, :24-90 reads it; the per-plane records
+// are written by src/io/OCFD_IO_mpi.c:605-701 and read by :563-604), native byte order:
+//   header record:  int32 12 | int32 Istep | float64 tt | int32 12
+//   then rho, u, v, w, T, each as NZ records, one per z plane (z ascending):
+//                   int32 NX*NY*8 | NX*NY float64 (x fastest, then y) | int32 NX*NY*8
+// i.e. Fortran sequential unformatted records.  Only the interior is stored.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ws_internal.h"
+
+using namespace ws;
+
+namespace {
+
+struct File {
+    FILE *f = nullptr;
+    ~File() { if (f) std::fclose(f); }
+};
+
+bool put_i32(FILE *f, int32_t v) { return std::fwrite(&v, sizeof(v), 1, f) == 1; }
+
+bool get_i32(FILE *f, int32_t *v) { return std::fread(v, sizeof(*v), 1, f) == 1; }
+
+int check_extents(int32_t nx, int32_t ny, int32_t nz)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0) return set_error(WS_ERR_INVALID, "restart: bad extents");
+    if ((long long)nx * ny * 8 > 0x7fffffffLL) return set_error(WS_ERR_INVALID, "restart: plane too large for an int32 record marker");
+    return WS_OK;
+}
+
+// interior of a ghosted device field <-> contiguous host array (x fastest)
+int field_to_host(const ws_field &fld, int nx, int ny, int nz, double *host)
+{
+    const size_t nyg = (size_t)ny + 2 * LAP;
+    for (int z = 0; z < nz; ++z) {
+        const double *src = fld.ptr + LAP + (size_t)fld.pitch * (LAP + nyg * (size_t)(z + LAP));
+        hipError_t e = hipMemcpy2D(host + (size_t)z * nx * ny, (size_t)nx * sizeof(double), src,
+                                   (size_t)fld.pitch * sizeof(double), (size_t)nx * sizeof(double), ny,
+                                   hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_error(e, "restart: device to host copy");
+    }
+    return WS_OK;
+}
+
+int host_to_field(const double *host, int nx, int ny, int nz, const ws_field &fld)
+{
+    const size_t nyg = (size_t)ny + 2 * LAP;
+    for (int z = 0; z < nz; ++z) {
+        double *dst = fld.ptr + LAP + (size_t)fld.pitch * (LAP + nyg * (size_t)(z + LAP));
+        hipError_t e = hipMemcpy2D(dst, (size_t)fld.pitch * sizeof(double), host + (size_t)z * nx * ny,
+                                   (size_t)nx * sizeof(double), (size_t)nx * sizeof(double), ny,
+                                   hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_error(e, "restart: host to device copy");
+    }
+    return WS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ws_restart_name(int32_t step, char *buf, int32_t buflen)
+{
+    if (!buf || buflen <= 0 || step < 0) return set_error(WS_ERR_INVALID, "restart name: bad arguments");
+    const int n = std::snprintf(buf, (size_t)buflen, "OCFD%08d.dat", step);
+    if (n < 0 || n >= buflen) return set_error(WS_ERR_INVALID, "restart name: buffer too small");
+    return WS_OK;
+}
+
+int ws_restart_write_host(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t istep, double tt,
+                          const double *const *fields)
+{
+    if (!path || !fields) return set_error(WS_ERR_INVALID, "restart write: null argument");
+    int r = check_extents(nx, ny, nz);
+    if (r) return r;
+    for (int m = 0; m < 5; ++m)
+        if (!fields[m]) return set_error(WS_ERR_INVALID, "restart write: null field");
+    File fh;
+    fh.f = std::fopen(path, "wb");
+    if (!fh.f) return set_error(WS_ERR_INVALID, std::string("restart write: cannot open ") + path);
+    const int32_t hdr = (int32_t)(sizeof(int32_t) + sizeof(double));
+    bool ok = put_i32(fh.f, hdr) && put_i32(fh.f, istep) && std::fwrite(&tt, sizeof(tt), 1, fh.f) == 1 &&
+              put_i32(fh.f, hdr);
+    const size_t plane = (size_t)nx * ny;
+    const int32_t rec = (int32_t)(plane * sizeof(double));
+    for (int m = 0; m < 5 && ok; ++m)
+        for (int z = 0; z < nz && ok; ++z)
+            ok = put_i32(fh.f, rec) && std::fwrite(fields[m] + (size_t)z * plane, sizeof(double), plane, fh.f) == plane &&
+                 put_i32(fh.f, rec);
+    if (!ok) return set_error(WS_ERR_INVALID, std::string("restart write: short write to ") + path);
+    if (std::fclose(fh.f) != 0) { fh.f = nullptr; return set_error(WS_ERR_INVALID, "restart write: close failed"); }
+    fh.f = nullptr;
+    return WS_OK;
+}
+
+int ws_restart_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, int32_t *istep, double *tt,
+                         double *const *fields)
+{
+    if (!path || !fields || !istep || !tt) return set_error(WS_ERR_INVALID, "restart read: null argument");
+    int r = check_extents(nx, ny, nz);
+    if (r) return r;
+    for (int m = 0; m < 5; ++m)
+        if (!fields[m]) return set_error(WS_ERR_INVALID, "restart read: null field");
+    File fh;
+    fh.f = std::fopen(path, "rb");
+    if (!fh.f) return set_error(WS_ERR_INVALID, std::string("restart read: cannot open ") + path);
+    const int32_t hdr = (int32_t)(sizeof(int32_t) + sizeof(double));
+    int32_t a = 0, b = 0;
+    if (!get_i32(fh.f, &a) || !get_i32(fh.f, istep) || std::fread(tt, sizeof(*tt), 1, fh.f) != 1 ||
+        !get_i32(fh.f, &b) || a != hdr || b != hdr)
+        return set_error(WS_ERR_INVALID, "restart read: bad header record");
+    const size_t plane = (size_t)nx * ny;
+    const int32_t rec = (int32_t)(plane * sizeof(double));
+    for (int m = 0; m < 5; ++m)
+        for (int z = 0; z < nz; ++z) {
+            if (!get_i32(fh.f, &a) || a != rec)
+                return set_error(WS_ERR_INVALID, "restart read: plane record does not match nx*ny of this grid");
+            if (std::fread(fields[m] + (size_t)z * plane, sizeof(double), plane, fh.f) != plane || !get_i32(fh.f, &b) ||
+                b != rec)
+                return set_error(WS_ERR_INVALID, "restart read: truncated plane record");
+        }
+    // the file must end here: a longer file belongs to a larger grid
+    if (std::fgetc(fh.f) != EOF) return set_error(WS_ERR_INVALID, "restart read: file longer than this grid's data");
+    return WS_OK;
+}
+
+int ws_save_restart(ws_ctx *c, const ws_state *st, int32_t istep, double tt, const char *path, hipStream_t s)
+{
+    if (!c || !st || !path) return set_error(WS_ERR_INVALID, "restart save: null argument");
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(e, "restart save: stream sync");
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<double> host(5 * n);
+    const ws_field fl[5] = {st->d, st->u, st->v, st->w, st->T};
+    for (int m = 0; m < 5; ++m) {
+        if (!fl[m].ptr || fl[m].pitch < (uint32_t)(nx + 2 * LAP)) return set_error(WS_ERR_INVALID, "restart save: bad field");
+        int r = field_to_host(fl[m], nx, ny, nz, host.data() + m * n);
+        if (r) return r;
+    }
+    const double *p[5] = {host.data(), host.data() + n, host.data() + 2 * n, host.data() + 3 * n, host.data() + 4 * n};
+    return ws_restart_write_host(path, nx, ny, nz, istep, tt, p);
+}
+
+int ws_load_restart(ws_ctx *c, ws_state *st, const char *path, int32_t *istep, double *tt, hipStream_t s)
+{
+    if (!c || !st || !path || !istep || !tt) return set_error(WS_ERR_INVALID, "restart load: null argument");
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<double> host(5 * n);
+    double *p[5] = {host.data(), host.data() + n, host.data() + 2 * n, host.data() + 3 * n, host.data() + 4 * n};
+    int r = ws_restart_read_host(path, nx, ny, nz, istep, tt, p);
+    if (r) return r;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(e, "restart load: stream sync");
+    const ws_field fl[5] = {st->d, st->u, st->v, st->w, st->T};
+    for (int m = 0; m < 5; ++m) {
+        if (!fl[m].ptr || fl[m].pitch < (uint32_t)(nx + 2 * LAP)) return set_error(WS_ERR_INVALID, "restart load: bad field");
+        if ((r = host_to_field(p[m], nx, ny, nz, fl[m]))) return r;
+    }
+    return WS_OK;
+}
+
+}  // extern "C"

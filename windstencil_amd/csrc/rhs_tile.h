@@ -66,100 +66,137 @@ struct Tile {
     __device__ static int fidx(int l, int f) { return l * F + f; }
 };
 
-// One tile (tile coordinates bx, by, bz within the box) of one direction, with the
-// block's LDS buffer sp (10 * QS + PAD doubles).  Every thread of the block calls it.
-template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX>
-__device__ __forceinline__ void rhs_tile_body(const KParams &k, const ws_field &d, const ws_field &u,
-                                              const ws_field &v, const ws_field &w, const ws_field &cc,
-                                              const ws_metric_fields &mf, const ws_soa &du, const Box &box,
-                                              int bx, int by, int bz, double *sp)
+// Where a tile lies: x0 always tiles x; the line axis is x (DIR 0), y (DIR 1) or z
+// (DIR 2).  Cells of the tile inside the box: lim_line along the line, lines
+// lo_across .. lim_across-1 across (only the first aligned y/z tile has lo_across > 0).
+struct TileGeo {
+    int x0, y0, z0;
+    int lim_line, lim_across, lo_across;
+};
+
+template <int DIR, int L, int W, int K>
+__device__ __forceinline__ TileGeo tile_geo(const Box &box, int bx, int by, int bz)
+{
+    using TL = Tile<DIR, L, W, K>;
+    TileGeo g;
+    if (DIR == 0) {
+        g.x0 = box.s[0] + bx * L;
+        g.y0 = box.s[1] + by * W;
+        g.z0 = box.s[2] + bz;
+    } else if (DIR == 1) {
+        g.x0 = box.s[0] / TL::XALIGN * TL::XALIGN + bx * W;
+        g.y0 = box.s[1] + by * L;
+        g.z0 = box.s[2] + bz;
+    } else {
+        g.x0 = box.s[0] / TL::XALIGN * TL::XALIGN + bx * W;
+        g.z0 = box.s[2] + by * L;
+        g.y0 = box.s[1] + bz;
+    }
+    g.lim_line = DIR == 0 ? box.e[0] - g.x0 : DIR == 1 ? box.e[1] - g.y0 : box.e[2] - g.z0;
+    g.lim_across = DIR == 0 ? box.e[1] - g.y0 : box.e[0] - g.x0;
+    g.lo_across = DIR == 0 ? 0 : box.s[0] - g.x0;
+    return g;
+}
+
+// The primitive values (rho, u, v, w, c) of this thread's phase-1 points.
+template <int DIR, int L, int W, int K>
+struct TileLoads {
+    double v[Tile<DIR, L, W, K>::ROUNDS][5];
+};
+
+// point r of this thread: line l, point p (x fastest in memory for every direction)
+template <int DIR, int L, int W, int K>
+__device__ __forceinline__ void tile_point(int r, int &l, int &p)
+{
+    using TL = Tile<DIR, L, W, K>;
+    const int pt = threadIdx.x + r * TL::NT;
+    if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }
+    else          { l = pt % W;      p = pt / W; }
+}
+
+template <int DIR, int L, int W, int K>
+__device__ __forceinline__ bool tile_point_ok(const TileGeo &g, int r, int l, int p)
+{
+    using TL = Tile<DIR, L, W, K>;
+    // a line needs points -LAP .. lim_line+LAP-1 of lines lo_across .. lim_across-1
+    return (int)threadIdx.x + r * TL::NT < TL::NPTS && l >= g.lo_across && l < g.lim_across &&
+           p < g.lim_line + 2 * LAP;
+}
+
+template <int DIR>
+__device__ __forceinline__ void tile_point_xyz(const TileGeo &g, int l, int p, int &x, int &y, int &z)
+{
+    if (DIR == 0)      { x = g.x0 - LAP + p; y = g.y0 + l; z = g.z0; }
+    else if (DIR == 1) { x = g.x0 + l; y = g.y0 - LAP + p; z = g.z0; }
+    else               { x = g.x0 + l; y = g.y0; z = g.z0 - LAP + p; }
+}
+
+// Phase 1a: issue the loads of this thread's points (nothing waits on them here).
+template <int DIR, int L, int W, int K>
+__device__ __forceinline__ void tile_load(int nyg, const ws_field &d, const ws_field &u, const ws_field &v,
+                                          const ws_field &w, const ws_field &cc, const TileGeo &g,
+                                          TileLoads<DIR, L, W, K> &ld)
+{
+    using TL = Tile<DIR, L, W, K>;
+#pragma unroll
+    for (int r = 0; r < TL::ROUNDS; ++r) {
+        int l, p, x, y, z;
+        tile_point<DIR, L, W, K>(r, l, p);
+        if (!tile_point_ok<DIR, L, W, K>(g, r, l, p)) continue;
+        tile_point_xyz<DIR>(g, l, p, x, y, z);
+        ld.v[r][0] = d.ptr[fidx(d.pitch, nyg, x, y, z)];
+        ld.v[r][1] = u.ptr[fidx(u.pitch, nyg, x, y, z)];
+        ld.v[r][2] = v.ptr[fidx(v.pitch, nyg, x, y, z)];
+        ld.v[r][3] = w.ptr[fidx(w.pitch, nyg, x, y, z)];
+        ld.v[r][4] = cc.ptr[fidx(cc.pitch, nyg, x, y, z)];
+    }
+}
+
+// Phases 1b-3 of one tile with the block's LDS buffer sp (10 * QS + PAD doubles):
+// split the loaded points into LDS, call next() (the persistent kernels issue the next
+// tile's loads there, into the registers just consumed), reconstruct the faces, then
+// write the flux difference of every cell.  Every thread of the block calls it.
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, typename Next>
+__device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_fields &mf, const ws_soa &du,
+                                             const TileGeo &g, TileLoads<DIR, L, W, K> &ld, double *sp,
+                                             Next next)
 {
     using TL = Tile<DIR, L, W, K>;
     const int t = threadIdx.x;
-
-    // tile origin: x0 always tiles x; the line axis is x (DIR 0), y (DIR 1) or z (DIR 2)
-    int x0, y0, z0;
-    if (DIR == 0) {
-        x0 = box.s[0] + bx * L;
-        y0 = box.s[1] + by * W;
-        z0 = box.s[2] + bz;
-    } else if (DIR == 1) {
-        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + bx * W;
-        y0 = box.s[1] + by * L;
-        z0 = box.s[2] + bz;
-    } else {
-        x0 = box.s[0] / TL::XALIGN * TL::XALIGN + bx * W;
-        z0 = box.s[2] + by * L;
-        y0 = box.s[1] + bz;
-    }
-    // cells of the tile that lie in the box, along the line and across lines
-    // (lines lo_across .. lim_across-1; only the first aligned y/z tile has lo_across > 0)
-    const int lim_line = DIR == 0 ? box.e[0] - x0 : DIR == 1 ? box.e[1] - y0 : box.e[2] - z0;
-    const int lim_across = DIR == 0 ? box.e[1] - y0 : box.e[0] - x0;
-    const int lo_across = DIR == 0 ? 0 : box.s[0] - x0;
-
     const double cA1 = k.A[3 * DIR + 0], cA2 = k.A[3 * DIR + 1], cA3 = k.A[3 * DIR + 2];
     const double cSn = k.Anorm[DIR];
     const double cK1 = k.Ak[3 * DIR + 0], cK2 = k.Ak[3 * DIR + 1], cK3 = k.Ak[3 * DIR + 2];
 
-    // ---- phase 1: split every point of the tile ----------------------------------------
-    // all loads of this thread's points are issued before any split arithmetic
-    double ld[TL::ROUNDS][5];
-    int lidx[TL::ROUNDS];
-    bool lok[TL::ROUNDS];
+    // ---- phase 1b: split every point of the tile into LDS --------------------------------
 #pragma unroll
     for (int r = 0; r < TL::ROUNDS; ++r) {
-        const int pt = t + r * TL::NT;
         int l, p;
-        if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }     // x fastest
-        else          { l = pt % W;      p = pt / W; }          // x fastest
-        // the line needs points -LAP .. lim_line+LAP-1 of lines lo_across .. lim_across-1
-        lok[r] = pt < TL::NPTS && l >= lo_across && l < lim_across && p < lim_line + 2 * LAP;
-        lidx[r] = TL::pidx(l, p);
-        if (lok[r]) {
-            int x, y, z;
-            if (DIR == 0)      { x = x0 - LAP + p; y = y0 + l; z = z0; }
-            else if (DIR == 1) { x = x0 + l; y = y0 - LAP + p; z = z0; }
-            else               { x = x0 + l; y = y0; z = z0 - LAP + p; }
-            const int nyg = k.nyg;
-            ld[r][0] = d.ptr[fidx(d.pitch, nyg, x, y, z)];
-            ld[r][1] = u.ptr[fidx(u.pitch, nyg, x, y, z)];
-            ld[r][2] = v.ptr[fidx(v.pitch, nyg, x, y, z)];
-            ld[r][3] = w.ptr[fidx(w.pitch, nyg, x, y, z)];
-            ld[r][4] = cc.ptr[fidx(cc.pitch, nyg, x, y, z)];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < TL::ROUNDS; ++r) {
-        if (!lok[r]) continue;
-        const double dd = ld[r][0], uu = ld[r][1], vv = ld[r][2], ww = ld[r][3], c = ld[r][4];
+        tile_point<DIR, L, W, K>(r, l, p);
+        if (!tile_point_ok<DIR, L, W, K>(g, r, l, p)) continue;
+        const double dd = ld.v[r][0], uu = ld.v[r][1], vv = ld.v[r][2], ww = ld.v[r][3], c = ld.v[r][4];
         double fp[5], fm[5];
         if (FIELDS) {
-            const int pt = t + r * TL::NT;
-            int l, p, x, y, z;
-            if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }
-            else          { l = pt % W;      p = pt / W; }
-            if (DIR == 0)      { x = x0 - LAP + p; y = y0 + l; z = z0; }
-            else if (DIR == 1) { x = x0 + l; y = y0 - LAP + p; z = z0; }
-            else               { x = x0 + l; y = y0; z = z0 - LAP + p; }
+            int x, y, z;
+            tile_point_xyz<DIR>(g, l, p, x, y, z);
             const int nyg = k.nyg;
             const ws_field &f1 = mf.A[3 * DIR + 0], &f2 = mf.A[3 * DIR + 1], &f3 = mf.A[3 * DIR + 2];
             const double A1 = f1.ptr[fidx(f1.pitch, nyg, x, y, z)];
             const double A2 = f2.ptr[fidx(f2.pitch, nyg, x, y, z)];
             const double A3 = f3.ptr[fidx(f3.pitch, nyg, x, y, z)];
             const double sn = sqrt(fma(A3, A3, fma(A2, A2, A1 * A1)));
-            const double r = 1.0 / sn;
-            split_fast(k, dd, uu, vv, ww, c, A1, A2, A3, sn, A1 * r, A2 * r, A3 * r, fp, fm);
+            const double rs = 1.0 / sn;
+            split_fast(k, dd, uu, vv, ww, c, A1, A2, A3, sn, A1 * rs, A2 * rs, A3 * rs, fp, fm);
         } else {
             split_fast(k, dd, uu, vv, ww, c, cA1, cA2, cA3, cSn, cK1, cK2, cK3, fp, fm);
         }
-        const int li = lidx[r];
+        const int li = TL::pidx(l, p);
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             sp[m * TL::QS + li] = fp[m];
             sp[(5 + m) * TL::QS + li] = fm[m];
         }
     }
+    next();
     __syncthreads();
 
     // ---- phase 3 prologue: accumulating launches read du; issue those loads now so
@@ -174,11 +211,11 @@ __device__ __forceinline__ void rhs_tile_body(const KParams &k, const ws_field &
             int l, i;
             if (DIR == 0) { i = c % L; l = c / L; }
             else          { l = c % W; i = c / W; }
-            if (c >= W * L || l < lo_across || l >= lim_across || i >= lim_line) continue;
+            if (c >= W * L || l < g.lo_across || l >= g.lim_across || i >= g.lim_line) continue;
             int cx, cy, cz;
-            if (DIR == 0)      { cx = x0 + i; cy = y0 + l; cz = z0; }
-            else if (DIR == 1) { cx = x0 + l; cy = y0 + i; cz = z0; }
-            else               { cx = x0 + l; cy = y0; cz = z0 + i; }
+            if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
+            else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
+            else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
             const double *dp = du.ptr + (size_t)(cx - LAP) +
                                (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
 #pragma unroll
@@ -189,7 +226,7 @@ __device__ __forceinline__ void rhs_tile_body(const KParams &k, const ws_field &
     // ---- phase 2: K faces of one line per thread, all five variables --------------------
     const int line = t / TL::T, seg = t % TL::T;
     const int f0 = seg * K;                               // first face of this thread
-    const bool active = (t < TL::NT) && (line >= lo_across) && (line < lim_across) && (f0 <= lim_line);
+    const bool active = (t < TL::NT) && (line >= g.lo_across) && (line < g.lim_across) && (f0 <= g.lim_line);
     double Fv[5][K];
     if (active) {
         const int base = TL::pidx(line, f0);
@@ -224,7 +261,7 @@ __device__ __forceinline__ void rhs_tile_body(const KParams &k, const ws_field &
     if (active) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            if (f0 + j > lim_line || f0 + j >= TL::F) break;
+            if (f0 + j > g.lim_line || f0 + j >= TL::F) break;
 #pragma unroll
             for (int m = 0; m < 5; ++m) sp[m * W * TL::F + TL::fidx(line, f0 + j)] = Fv[m][j];
         }
@@ -240,11 +277,11 @@ __device__ __forceinline__ void rhs_tile_body(const KParams &k, const ws_field &
         int l, i;
         if (DIR == 0) { i = c % L; l = c / L; }
         else          { l = c % W; i = c / W; }
-        if (l < lo_across || l >= lim_across || i >= lim_line) continue;
+        if (l < g.lo_across || l >= g.lim_across || i >= g.lim_line) continue;
         int cx, cy, cz;
-        if (DIR == 0)      { cx = x0 + i; cy = y0 + l; cz = z0; }
-        else if (DIR == 1) { cx = x0 + l; cy = y0 + i; cz = z0; }
-        else               { cx = x0 + l; cy = y0; cz = z0 + i; }
+        if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
+        else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
+        else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
         const double aj = FIELDS ? mf.ajac.ptr[fidx(mf.ajac.pitch, k.nyg, cx, cy, cz)] : k.ajac;
         const double scale = aj * scale0;
         double *dp = du.ptr + (size_t)(cx - LAP) +
@@ -266,8 +303,45 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
 {
     using TL = Tile<DIR, L, W, K>;
     __shared__ double sp[10 * TL::QS + TL::PAD];
-    rhs_tile_body<DIR, L, W, K, FIELDS, ACC, MIX>(k, d, u, v, w, cc, mf, du, box, blockIdx.x, blockIdx.y,
-                                                  blockIdx.z, sp);
+    const TileGeo g = tile_geo<DIR, L, W, K>(box, blockIdx.x, blockIdx.y, blockIdx.z);
+    TileLoads<DIR, L, W, K> ld;
+    tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
+    tile_compute<DIR, L, W, K, FIELDS, ACC, MIX>(k, mf, du, g, ld, sp, [] {});
+}
+
+// Persistent form for one box: the resident work-groups each walk a contiguous run of
+// the tiles in launch order (x tiles fastest), so neighbouring tiles, which share halo
+// lines, run on the same work-group and XCD.  With PREFETCH the next tile's loads are
+// issued as soon as this tile's split has consumed the load registers.
+template <int DIR, int L, int W, int K, bool ACC, bool MIX, bool PREFETCH>
+__global__ void __launch_bounds__(320)
+k_rhs_persist(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_soa du, Box box,
+              int gx, int gy, int ntiles, int chunk)
+{
+    using TL = Tile<DIR, L, W, K>;
+    __shared__ double sp[10 * TL::QS + TL::PAD];
+    const int first = blockIdx.x * chunk;
+    const int last = min(first + chunk, ntiles);
+    if (first >= last) return;
+    const ws_metric_fields nomf{};
+    auto geo = [&](int i) {
+        const int r = i / gx;
+        return tile_geo<DIR, L, W, K>(box, i - r * gx, r % gy, r / gy);
+    };
+    TileLoads<DIR, L, W, K> ld;
+    TileGeo g = geo(first);
+    if (PREFETCH) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
+    for (int i = first; i < last; ++i) {
+        if (!PREFETCH) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
+        const bool more = i + 1 < last;
+        const TileGeo gn = more ? geo(i + 1) : g;
+        auto next = [&] {
+            if (PREFETCH && more) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, gn, ld);
+        };
+        tile_compute<DIR, L, W, K, false, ACC, MIX>(k, nomf, du, g, ld, sp, next);
+        __syncthreads();   // the next tile reuses the LDS buffer
+        g = gn;
+    }
 }
 
 // Batched form: the tiles of one direction over every block of a batch, in one
@@ -275,7 +349,7 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
 // of the tile list, so neighbouring tiles (which share halo lines) run on the same
 // work-group, hence the same XCD and L2.  Each block brings its own fields, extents
 // and spacing; the physics constants and metrics are the batch's.
-template <int DIR, int L, int W, int K, bool ACC, bool MIX>
+template <int DIR, int L, int W, int K, bool ACC, bool MIX, bool PREFETCH>
 __global__ void __launch_bounds__(320)
 k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, int chunk)
 {
@@ -283,26 +357,107 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
     __shared__ double sp[10 * TL::QS + TL::PAD];
     const int first = blockIdx.x * chunk;
     const int last = min(first + chunk, ntiles);
+    if (first >= last) return;
     const ws_metric_fields nomf{};
+    auto box_of = [](const BatchBlock &b) {
+        Box box;
+        box.s[0] = box.s[1] = box.s[2] = LAP;
+        box.e[0] = b.n[0] + LAP; box.e[1] = b.n[1] + LAP; box.e[2] = b.n[2] + LAP;
+        return box;
+    };
+    // tile entries are uniform across the work-group: keep them (and the geometry
+    // derived from them) in scalar registers
+    auto tile_at = [&](int i) {
+        const int4 v = tiles[i];
+        return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                         __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+    };
+    TileLoads<DIR, L, W, K> ld;
+    int4 tl = tile_at(first);
+    TileGeo g = tile_geo<DIR, L, W, K>(box_of(blocks[tl.x]), tl.y, tl.z, tl.w);
+    if (PREFETCH) {
+        const BatchBlock &b = blocks[tl.x];
+        tile_load<DIR, L, W, K>(b.n[1] + 2 * LAP, b.d, b.u, b.v, b.w, b.cc, g, ld);
+    }
     for (int i = first; i < last; ++i) {
-        const int4 tl = tiles[i];
         const BatchBlock &b = blocks[tl.x];
         KParams kb = k;
         kb.nx = b.n[0]; kb.ny = b.n[1]; kb.nz = b.n[2];
         kb.nyg = b.n[1] + 2 * LAP; kb.nzg = b.n[2] + 2 * LAP;
         kb.rh[DIR] = b.rh[DIR];
-        Box box;
-        box.s[0] = box.s[1] = box.s[2] = LAP;
-        box.e[0] = b.n[0] + LAP; box.e[1] = b.n[1] + LAP; box.e[2] = b.n[2] + LAP;
-        rhs_tile_body<DIR, L, W, K, false, ACC, MIX>(kb, b.d, b.u, b.v, b.w, b.cc, nomf, b.du, box, tl.y, tl.z,
-                                                     tl.w, sp);
+        if (!PREFETCH) tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
+        const bool more = i + 1 < last;
+        const int4 tn = more ? tile_at(i + 1) : tl;
+        const TileGeo gn = tile_geo<DIR, L, W, K>(box_of(blocks[tn.x]), tn.y, tn.z, tn.w);
+        // with PREFETCH the next tile's loads are issued as soon as this tile's split has
+        // consumed the load registers, and complete during its reconstruction
+        auto next = [&] {
+            if (PREFETCH && more) {
+                const BatchBlock &bn = blocks[tn.x];
+                tile_load<DIR, L, W, K>(bn.n[1] + 2 * LAP, bn.d, bn.u, bn.v, bn.w, bn.cc, gn, ld);
+            }
+        };
+        tile_compute<DIR, L, W, K, false, ACC, MIX>(kb, nomf, b.du, g, ld, sp, next);
         __syncthreads();   // the next tile reuses the LDS buffer
+        tl = tn;
+        g = gn;
     }
+}
+
+// Batched form with one tile per work-group (grid = tile count): no loop-carried
+// state, so it needs no more registers than the single-box kernel.
+template <int DIR, int L, int W, int K, bool ACC, bool MIX>
+__global__ void __launch_bounds__(320)
+k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles)
+{
+    using TL = Tile<DIR, L, W, K>;
+    __shared__ double sp[10 * TL::QS + TL::PAD];
+    const int4 tl = tiles[blockIdx.x];
+    const BatchBlock &b = blocks[tl.x];
+    KParams kb = k;
+    kb.nx = b.n[0]; kb.ny = b.n[1]; kb.nz = b.n[2];
+    kb.nyg = b.n[1] + 2 * LAP; kb.nzg = b.n[2] + 2 * LAP;
+    kb.rh[DIR] = b.rh[DIR];
+    Box box;
+    box.s[0] = box.s[1] = box.s[2] = LAP;
+    box.e[0] = b.n[0] + LAP; box.e[1] = b.n[1] + LAP; box.e[2] = b.n[2] + LAP;
+    const TileGeo g = tile_geo<DIR, L, W, K>(box, tl.y, tl.z, tl.w);
+    TileLoads<DIR, L, W, K> ld;
+    tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
+    const ws_metric_fields nomf{};
+    tile_compute<DIR, L, W, K, false, ACC, MIX>(kb, nomf, b.du, g, ld, sp, [] {});
+}
+
+// work-groups of a persistent kernel that fit on the device at once
+template <typename Kern>
+int resident_of(Kern kern, int threads)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0) != hipSuccess || per <= 0) per = 1;
+    return cus * per;
+}
+
+template <int DIR, int L, int W, int K, bool ACC, bool MIX, bool PF>
+void launch_persist(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_soa du,
+                    const Box &box, int gx, int gy, int gz, hipStream_t s)
+{
+    using TL = Tile<DIR, L, W, K>;
+    static const int resident = resident_of(k_rhs_persist<DIR, L, W, K, ACC, MIX, PF>, TL::NT);
+    const int n = gx * gy * gz;
+    if (n <= 0) return;
+    int grid = resident < n ? resident : n;
+    const int chunk = (n + grid - 1) / grid;
+    grid = (n + chunk - 1) / chunk;
+    hipLaunchKernelGGL((k_rhs_persist<DIR, L, W, K, ACC, MIX, PF>), dim3(grid), dim3(TL::NT), 0, s, k, d, u, v, w,
+                       cc, du, box, gx, gy, n, chunk);
 }
 
 template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX>
 void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                const ws_metric_fields &mf, ws_soa du, const Box &box, hipStream_t s)
+                const ws_metric_fields &mf, ws_soa du, const Box &box, int sched, hipStream_t s)
 {
     using TL = Tile<DIR, L, W, K>;
     const int nx = box.e[0] - box.s[0], ny = box.e[1] - box.s[1], nz = box.e[2] - box.s[2];
@@ -312,78 +467,82 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     if (DIR == 0) grid = dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
     else if (DIR == 1) grid = dim3((nxa + W - 1) / W, (ny + L - 1) / L, nz);
     else grid = dim3((nxa + W - 1) / W, (nz + L - 1) / L, ny);
-    hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX>), grid, dim3(TL::NT), 0, s, k, d, u, v, w, cc, mf,
-                       du, box);
+    if (FIELDS || sched == WS_SCHED_GRID) {
+        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX>), grid, dim3(TL::NT), 0, s, k, d, u, v, w,
+                           cc, mf, du, box);
+    } else if (sched == WS_SCHED_PREFETCH) {
+        launch_persist<DIR, L, W, K, ACC, MIX, true>(k, d, u, v, w, cc, du, box, grid.x, grid.y, grid.z, s);
+    } else {
+        launch_persist<DIR, L, W, K, ACC, MIX, false>(k, d, u, v, w, cc, du, box, grid.x, grid.y, grid.z, s);
+    }
 }
 
 template <int L0, int W0, int L1, int W1, int K, bool FIELDS, bool MIX>
 void launch_all(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                const ws_metric_fields &mf, ws_soa du, const Box &box, int accumulate, hipStream_t s)
+                const ws_metric_fields &mf, ws_soa du, const Box &box, int accumulate, int sched, hipStream_t s)
 {
-    if (accumulate) launch_dir<0, L0, W0, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, s);
-    else launch_dir<0, L0, W0, K, FIELDS, false, MIX>(k, d, u, v, w, cc, mf, du, box, s);
-    launch_dir<1, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, s);
-    launch_dir<2, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, s);
+    if (accumulate) launch_dir<0, L0, W0, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+    else launch_dir<0, L0, W0, K, FIELDS, false, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+    launch_dir<1, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+    launch_dir<2, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
 }
 
 template <int S, bool FIELDS, bool MIX>
 void launch_shape_n(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                    const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, hipStream_t s)
+                    const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, hipStream_t s)
 {
     constexpr TileShapeDef D = kTileShapes[S];
-    launch_all<D.L0, D.W0, D.L1, D.W1, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s);
+    launch_all<D.L0, D.W0, D.L1, D.W1, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s);
 }
 
 template <bool FIELDS, bool MIX>
 void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                  const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, hipStream_t s)
+                  const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, hipStream_t s)
 {
     switch (shape) {
-    case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s); break;
-    case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s); break;
-    default: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, s); break;
+    case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
+    case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
+    default: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
     }
 }
 
 // ---- batched launches ------------------------------------------------------------------
 
-// work-groups of k_rhs_batch that fit on the device at once (the persistent grid)
-template <int DIR, int L, int W, int K, bool ACC, bool MIX>
-int resident_groups()
-{
-    static const int n = [] {
-        int dev = 0, cus = 0, per = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rhs_batch<DIR, L, W, K, ACC, MIX>,
-                                                         Tile<DIR, L, W, K>::NT, 0) != hipSuccess || per <= 0)
-            per = 1;
-        return cus * per;
-    }();
-    return n;
-}
-
-template <int DIR, int L, int W, int K, bool ACC, bool MIX>
+template <int DIR, int L, int W, int K, bool ACC, bool MIX, bool PF>
 void launch_batch_dir(const KParams &k, const BatchTiles &bt, hipStream_t s)
 {
     const int n = bt.ntiles[DIR];
     if (n <= 0) return;
-    int grid = resident_groups<DIR, L, W, K, ACC, MIX>();
-    if (grid > n) grid = n;
+    if (bt.schedule == WS_SCHED_GRID) {
+        hipLaunchKernelGGL((k_rhs_batch_grid<DIR, L, W, K, ACC, MIX>), dim3(n), dim3(Tile<DIR, L, W, K>::NT), 0, s,
+                           k, bt.blocks, bt.tiles[DIR]);
+        return;
+    }
+    static const int resident = resident_of(k_rhs_batch<DIR, L, W, K, ACC, MIX, PF>, Tile<DIR, L, W, K>::NT);
+    int grid = resident < n ? resident : n;
     const int chunk = (n + grid - 1) / grid;
     grid = (n + chunk - 1) / chunk;
-    hipLaunchKernelGGL((k_rhs_batch<DIR, L, W, K, ACC, MIX>), dim3(grid), dim3(Tile<DIR, L, W, K>::NT), 0, s, k,
-                       bt.blocks, bt.tiles[DIR], n, chunk);
+    hipLaunchKernelGGL((k_rhs_batch<DIR, L, W, K, ACC, MIX, PF>), dim3(grid), dim3(Tile<DIR, L, W, K>::NT), 0, s,
+                       k, bt.blocks, bt.tiles[DIR], n, chunk);
 }
 
-template <int S, bool MIX>
+template <int S, bool MIX, bool PF>
 void launch_batch_shape_n(const KParams &k, const BatchTiles &bt, hipStream_t s)
 {
     constexpr TileShapeDef D = kTileShapes[S];
-    launch_batch_dir<0, D.L0, D.W0, D.K, false, MIX>(k, bt, s);
-    launch_batch_dir<1, D.L1, D.W1, D.K, true, MIX>(k, bt, s);
-    launch_batch_dir<2, D.L1, D.W1, D.K, true, MIX>(k, bt, s);
+    launch_batch_dir<0, D.L0, D.W0, D.K, false, MIX, PF>(k, bt, s);
+    launch_batch_dir<1, D.L1, D.W1, D.K, true, MIX, PF>(k, bt, s);
+    launch_batch_dir<2, D.L1, D.W1, D.K, true, MIX, PF>(k, bt, s);
+}
+
+template <bool MIX, bool PF>
+void launch_batch_pf(const KParams &k, const BatchTiles &bt, hipStream_t s)
+{
+    switch (bt.shape) {
+    case 0: launch_batch_shape_n<0, MIX, PF>(k, bt, s); break;
+    case 1: launch_batch_shape_n<1, MIX, PF>(k, bt, s); break;
+    default: launch_batch_shape_n<2, MIX, PF>(k, bt, s); break;
+    }
 }
 
 template <bool MIX>
@@ -397,9 +556,9 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
     const int shape = c.tile_shape;
     if (m) {
         mf = *m;
-        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, s);
+        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, s);
     } else {
-        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, s);
+        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, MIX ? "mixed-precision RHS launch" : "tiled RHS launch");
@@ -409,11 +568,8 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
 template <bool MIX>
 int launch_rhs_batch_tile(const KParams &k, const BatchTiles &bt, hipStream_t s)
 {
-    switch (bt.shape) {
-    case 0: launch_batch_shape_n<0, MIX>(k, bt, s); break;
-    case 1: launch_batch_shape_n<1, MIX>(k, bt, s); break;
-    default: launch_batch_shape_n<2, MIX>(k, bt, s); break;
-    }
+    if (bt.schedule == WS_SCHED_PREFETCH) launch_batch_pf<MIX, true>(k, bt, s);
+    else launch_batch_pf<MIX, false>(k, bt, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, MIX ? "batched mixed-precision RHS launch" : "batched RHS launch");
     return WS_OK;

@@ -44,6 +44,8 @@ struct Ctx {
     hipStream_t capture_stream = nullptr;
     // tile shape of the tiled kernels, FP64 and mixed (WS_TILE_*)
     int tile_shape = WS_TILE_30x16_K2;
+    // how the tiled kernels walk their tiles (WS_SCHED_*)
+    int schedule = WS_SCHED_GRID;
 };
 
 KParams make_kparams(const ws_config &cfg);
@@ -61,13 +63,15 @@ struct BatchBlock {
 // one list per direction.
 struct BatchTiles {
     int shape;
+    int schedule;        // WS_BATCH_*
     const BatchBlock *blocks;
     const int4 *tiles[3];
     int ntiles[3];
 };
 
-// tile shape the autotuner found best for these extents and precision, or -1
-int remembered_shape(int nx, int ny, int nz, int precision);
+// (tile shape, schedule) the autotuner found best for these extents and precision;
+// false if none
+bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule);
 
 // error reporting (thread-local message)
 int set_error(int code, const std::string &msg);
