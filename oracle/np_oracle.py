@@ -158,3 +158,49 @@ def euler_flux(d, u, v, w, p, A1, A2, A3, gamma):
     V = A1 * u + A2 * v + A3 * w
     E = p / (gamma - 1.0) + 0.5 * d * (u * u + v * v + w * w)
     return [d * V, d * u * V + A1 * p, d * v * V + A2 * p, d * w * V + A3 * p, (E + p) * V]
+
+
+# ---- viscous terms -----------------------------------------------------------------------
+# Central first derivatives as antisymmetric weight tables (CD6: OCFD_Schemes.cpp:99-105,
+# CD8: :130-138): d f/dx_i ~ sum_q c_q (f_{i+q} - f_{i-q}) / h.
+_CD = {6: (45.0 / 60.0, -9.0 / 60.0, 1.0 / 60.0),
+       8: (672.0 / 840.0, -168.0 / 840.0, 32.0 / 840.0, -3.0 / 840.0)}
+
+
+def _dper(a, axis, h, order):
+    """Periodic central derivative of an interior array (nz, ny, nx) along axis (2 = x)."""
+    out = np.zeros_like(a)
+    for q, c in enumerate(_CD[order], start=1):
+        out += c * (np.roll(a, -q, axis=axis) - np.roll(a, q, axis=axis))
+    return out / h
+
+
+def viscous_rhs(prims_int, metrics_int, ajac_int, h, gamma, ama, re, pr, ref_t, order=6):
+    """Viscous terms (du components 1..4) of periodic interior fields: u, v, w, T arrays
+    (nz, ny, nx) in prims_int, the nine metric terms and the Jacobian as interior arrays.
+    Written with tensor algebra: grad[i][j] = sum_d A_d[j] * d(f_i)/dxi_d, the stress
+    tau = mu (grad + grad^T - 2/3 div I), the heat flux k grad T with k = mu Cp/Pr, and for
+    each direction d the flux F_d = J (A_d . tau, A_d . (tau u + k grad T)), whose
+    periodic central derivative, times J, is added."""
+    u, v, w, T = prims_int
+    vel = (u, v, w)
+    axes = (2, 1, 0)                     # computational x, y, z directions of the arrays
+    tsb = 110.4 / ref_t
+    mu = (1.0 + 110.4 / ref_t) / re * np.sqrt(T ** 3) / (tsb + T)
+    kappa = mu * (1.0 / ((gamma - 1.0) * ama * ama)) / pr
+    A = [[metrics_int[3 * d + j] for j in range(3)] for d in range(3)]     # A[d][j]
+    dvel = [[_dper(f, axes[d], h[d], order) for d in range(3)] for f in vel]  # d f_i / d xi_d
+    dT = [_dper(T, axes[d], h[d], order) for d in range(3)]
+    grad = [[sum(A[d][j] * dvel[i][d] for d in range(3)) for j in range(3)] for i in range(3)]
+    gT = [sum(A[d][j] * dT[d] for d in range(3)) for j in range(3)]
+    div = grad[0][0] + grad[1][1] + grad[2][2]
+    tau = [[mu * (grad[i][j] + grad[j][i] - (2.0 / 3.0 * div if i == j else 0.0)) for j in range(3)]
+           for i in range(3)]
+    heat = [sum(vel[i] * tau[i][j] for i in range(3)) + kappa * gT[j] for j in range(3)]
+    du = np.zeros((5,) + u.shape)
+    for d in range(3):
+        a = [A[d][j] * ajac_int for j in range(3)]
+        flux = [sum(a[j] * tau[i][j] for j in range(3)) for i in range(3)] + [sum(a[j] * heat[j] for j in range(3))]
+        for m in range(4):
+            du[m + 1] += ajac_int * _dper(flux[m], axes[d], h[d], order)
+    return du

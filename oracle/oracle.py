@@ -44,6 +44,10 @@ def lib():
         L.orc_weno7_symbo_p_parts.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_split_sw.argtypes = [ctypes.c_double] * 10 + [_dp, _dp]
         L.orc_cfl_dt.restype = ctypes.c_double
+        L.orc_cd6.restype = ctypes.c_double
+        L.orc_cd6.argtypes = [_dp]
+        L.orc_cd8.restype = ctypes.c_double
+        L.orc_cd8.argtypes = [_dp]
         _lib = L
     return _lib
 
@@ -143,6 +147,33 @@ def du_comput(p, d, u, v, w, T, metrics=None, ajac=None):
     lib().orc_du_comput(ctypes.byref(p), _ptr(d), _ptr(u), _ptr(v), _ptr(w), _ptr(T), _ptr(cc),
                         _metric_ptrs(metrics), _ptr(ajac), _ptr(du))
     return du, cc
+
+
+def cd(s):
+    """Central first-derivative stencil (CD6 for 7 points, CD8 for 9), before 1/h."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    return lib().orc_cd6(_ptr(s)) if len(s) == 7 else lib().orc_cd8(_ptr(s))
+
+
+def amu(p, T, re, ref_t):
+    out = np.zeros((p.nz, p.ny, p.nx))
+    lib().orc_amu(ctypes.byref(p), ctypes.c_double(re), ctypes.c_double(ref_t), _ptr(T), _ptr(out))
+    return out
+
+
+def viscous_rhs(p, u, v, w, T, re, pr, ref_t, order=6, metrics=None, ajac=None, du=None):
+    """Viscous terms added to du[1..4] (a zero du unless given) from ghosted u, v, w, T
+    whose periodic ghosts are filled."""
+    if metrics is None:
+        metrics = metrics_cartesian(p)
+    if ajac is None:
+        ajac = np.ones_like(u)
+    if du is None:
+        du = np.zeros((5, p.nz, p.ny, p.nx))
+    lib().orc_viscous_rhs(ctypes.byref(p), ctypes.c_double(re), ctypes.c_double(pr), ctypes.c_double(ref_t),
+                          ctypes.c_int(order), _ptr(u), _ptr(v), _ptr(w), _ptr(T), _metric_ptrs(metrics),
+                          _ptr(ajac), _ptr(du))
+    return du
 
 
 def time_advance(p, krk, f, fn, du, dt):

@@ -75,6 +75,30 @@ double orc_cfl_dt(const orc_params *p, double cfl,
 void orc_rk3_run(const orc_params *p, int nsteps, double dt,
                  double *f, double *d, double *u, double *v, double *w, double *T);
 
+/* ---- viscous terms (du_vis_Jacobian3d, src/solver/OCFD_Stream.cpp:229-276) ---- */
+
+/* Central first derivative of one stencil, src/kernels/OCFD_Schemes.cpp:99-105 (CD6,
+ * s[0..6] centred on s[3]) and :130-138 (CD8, s[0..8] centred on s[4]); the caller
+ * divides by h (put_d0_kernel, :80-97). */
+double orc_cd6(const double *s7);
+double orc_cd8(const double *s9);
+
+/* Sutherland viscosity on the interior (get_Amu_kernal, src/kernels/commen_kernel.cpp:123-141):
+ * amu = amu_C0 sqrt(T^3)/(Tsb + T), amu_C0 = (1 + 110.4/Ref_T)/Re, Tsb = 110.4/Ref_T
+ * (opencfd_para_init, src/runtime/OCFD_init.cpp:379-384).  amu: interior array. */
+void orc_amu(const orc_params *p, double re, double ref_t, const double *T, double *amu);
+
+/* Viscous RHS accumulated into du components 1..4 (momentum x, y, z, energy), from ghosted
+ * u, v, w, T whose periodic ghosts are filled: gradients by central differences of order
+ * 6 or 8 (du_viscous_Jacobian3d_init, OCFD_NS_Jacobian3d.cpp:455-474), the stress and heat
+ * flux per direction (vis_flux_ker, :477-620, with Cp/Pr = vis_flux_init_c), the periodic
+ * exchange of those fluxes in the direction, their central derivative and du += Ajac * that
+ * (du_viscous_Jacobian3d_[xyz]_final, :635-789; YF_Pe_XF, commen_kernel.cpp:159-170),
+ * directions x, y, z in turn. */
+void orc_viscous_rhs(const orc_params *p, double re, double pr, double ref_t, int order,
+                     const double *u, const double *v, const double *w, const double *T,
+                     const double *const metrics[9], const double *ajac, double *du);
+
 /* Internals of one P reconstruction (for first-principles tests): smoothness
  * indicators after the downwind max, the unnormalised product-form weights
  * d_k*prod_{j!=k}(eps+S_j) and the five candidate values q_k (before the
