@@ -113,7 +113,9 @@ typedef struct ws_config {
     int32_t precision;           /* WS_PREC_FP64 or WS_PREC_MIXED */
     int32_t kernel;              /* WS_KERNEL_TILED or WS_KERNEL_ANCHOR */
     int32_t bc_planes;           /* 1: OCFD_bc re-derivation of the boundary planes after each stage */
-    int32_t reserved;
+    int32_t viscous;             /* 0: inviscid RHS only; 6 or 8: ws_du_comput adds the viscous terms with
+                                    CD6 / CD8 central differences (Scheme_vis_ID 203 / 204) */
+    double re, pr, ref_t;        /* Re, Pr, Ref_T of the viscous terms (Sutherland viscosity) */
 } ws_config;
 
 typedef struct ws_ctx ws_ctx;
@@ -215,6 +217,17 @@ int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipSt
  * stream.  No counterpart in the reference. */
 int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
 
+/* Viscous terms of the RHS added to du components 1..4 (the reference's du_vis_Jacobian3d,
+ * src/solver/OCFD_Stream.cpp:229-276, with du_viscous_Jacobian3d_{init,[xyz]_init,[xyz]_final},
+ * src/solver/OCFD_NS_Jacobian3d.cpp:455-789): Sutherland viscosity (get_Amu,
+ * src/kernels/commen_kernel.cpp:123-141), central-difference gradients of u, v, w, T, the
+ * stress and heat flux of each direction, their periodic exchange and central derivative,
+ * du += Ajac * that.  u, v, w, T must have their periodic ghosts filled; cfg.viscous gives
+ * the order (6 or 8).  Uses a device scratch of 12 ghosted fields owned by the context,
+ * allocated at the first call. */
+int ws_viscous_rhs(ws_ctx *ctx, ws_field u, ws_field v, ws_field w, ws_field T, const ws_metric_fields *metrics,
+                   ws_soa du, hipStream_t stream);
+
 /* ---- time integration ---------------------------------------------------- */
 
 /* One SSP-RK3 step (the reference loop body, src/solver/OCFD_NS_Solver.cpp:64-90, with
@@ -250,6 +263,19 @@ int ws_save_restart(ws_ctx *ctx, const ws_state *st, int32_t istep, double tt, c
                     hipStream_t stream);
 int ws_load_restart(ws_ctx *ctx, ws_state *st, const char *path, int32_t *istep, double *tt,
                     hipStream_t stream);
+
+/* The reference's metric file OCFD3d-Jacobi.dat (read by Init_Jacobian3d,
+ * src/solver/OCFD_Comput_Jacobian3d.cpp:138-205): no header; x, y, z, Akx, Aky, Akz, Aix,
+ * Aiy, Aiz, Asx, Asy, Asz, Ajac as NZ plane records each (same records as above).
+ * Host arrays: fields[0..12] in that order, nx*ny*nz each, x fastest. */
+int ws_jacobi_write_host(const char *path, int32_t nx, int32_t ny, int32_t nz, const double *const *fields);
+int ws_jacobi_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, double *const *fields);
+/* Load the metric terms into metrics->A[0..8] (Akx..Asz) and metrics->ajac, and the mesh
+ * coordinates into coords[0..2] unless coords is NULL, then fill their periodic face
+ * ghosts (the reference's exchange_boundary_xyz after the read).  The fields are
+ * ghosted device fields of the context's extents. */
+int ws_load_jacobi(ws_ctx *ctx, const char *path, const ws_metric_fields *metrics, const ws_field *coords,
+                   hipStream_t stream);
 
 /* ---- batches of blocks ------------------------------------------------------
  * Many independent periodic blocks of uneven size advanced together.  No counterpart

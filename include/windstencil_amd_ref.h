@@ -37,7 +37,11 @@ void du_invis_Jacobian3d_init(ws_job job_in, hipStream_t *stream);
 void du_invis_Jacobian3d_x(ws_job job_in, ws_soa *fp, ws_soa *fm, hipStream_t *stream);
 /* src/solver/OCFD_Stream.cpp:185-227, include/windstencil/OCFD_Stream.h:19 */
 void *du_invis_Jacobian3d(void *pthread_id);
-/* src/solver/OCFD_Stream.cpp:27-52, OCFD_Stream.h:10 (inviscid part) */
+/* src/solver/OCFD_Stream.cpp:229-276, OCFD_Stream.h:25: the viscous terms added to du
+ * (requires cfg.viscous = 6 or 8; ghosts of u, v, w, T filled by the inviscid part) */
+void *du_vis_Jacobian3d(void *pthread_id);
+/* src/solver/OCFD_Stream.cpp:27-52, OCFD_Stream.h:10: the inviscid part, then the viscous
+ * part when cfg.viscous is set */
 void du_comput(int KRK);
 /* src/kernels/commen_kernel.cpp:112-119, include/windstencil/commen_kernel.h:22 */
 void get_duvwT(void);

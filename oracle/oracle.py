@@ -23,7 +23,8 @@ class OrcParams(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
                 ("gamma", ctypes.c_double), ("ama", ctypes.c_double), ("epsl_sw", ctypes.c_double),
                 ("hx", ctypes.c_double), ("hy", ctypes.c_double), ("hz", ctypes.c_double),
-                ("bc_planes", ctypes.c_int), ("nthreads", ctypes.c_int)]
+                ("bc_planes", ctypes.c_int), ("nthreads", ctypes.c_int), ("viscous", ctypes.c_int),
+                ("re", ctypes.c_double), ("pr", ctypes.c_double), ("ref_t", ctypes.c_double)]
 
 
 def build():
@@ -57,10 +58,11 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def params(nx, ny, nz, gamma=1.4, ama=1.0, epsl_sw=1e-3, h=None, bc_planes=1, nthreads=0):
+def params(nx, ny, nz, gamma=1.4, ama=1.0, epsl_sw=1e-3, h=None, bc_planes=1, nthreads=0, viscous=0,
+           re=1000.0, pr=0.72, ref_t=288.15):
     if h is None:
         h = (1.0 / (nx - 1.0), 1.0 / (ny - 1.0), 1.0 / (nz - 1.0))
-    return OrcParams(nx, ny, nz, gamma, ama, epsl_sw, h[0], h[1], h[2], bc_planes, nthreads)
+    return OrcParams(nx, ny, nz, gamma, ama, epsl_sw, h[0], h[1], h[2], bc_planes, nthreads, viscous, re, pr, ref_t)
 
 
 def weno_p(s):

@@ -271,6 +271,8 @@ void orc_du_comput(const orc_params *p, double *d, double *u, double *v, double 
     orc_ghost_fill_periodic(p, T);
     orc_sound_speed(p, T, cc);
     orc_inviscid_rhs(p, d, u, v, w, cc, metrics, ajac, du);
+    if (p->viscous == 6 || p->viscous == 8)
+        orc_viscous_rhs(p, p->re, p->pr, p->ref_t, p->viscous, u, v, w, T, metrics, ajac, du);
 }
 
 /* ---------------------------------------------------------------- time */

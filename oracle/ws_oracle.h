@@ -27,6 +27,8 @@ typedef struct orc_params {
     double hx, hy, hz;       /* grid spacings, 1/(N_global-1) in the reference */
     int bc_planes;           /* 1: re-derive conserved vars on the 6 boundary planes after each stage (OCFD_bc) */
     int nthreads;            /* OpenMP threads, 0 = runtime default */
+    int viscous;             /* 0: inviscid RHS only; 6 or 8: add the viscous terms with CD6 / CD8 */
+    double re, pr, ref_t;    /* Re, Pr, Ref_T of the viscous terms */
 } orc_params;
 
 /* scalar kernels (for closed-form tests) */
@@ -53,7 +55,8 @@ void orc_inviscid_rhs(const orc_params *p,
                       const double *const metrics[9], const double *ajac, double *du);
 
 /* Full RHS of one RK stage from the current primitives (du_comput):
- * zero du, periodic ghost fill of d,u,v,w,T, sound speed, inviscid RHS. */
+ * zero du, periodic ghost fill of d,u,v,w,T, sound speed, inviscid RHS, then the
+ * viscous terms when p->viscous is 6 or 8 (src/solver/OCFD_Stream.cpp:27-52). */
 void orc_du_comput(const orc_params *p, double *d, double *u, double *v, double *w, double *T, double *cc,
                    const double *const metrics[9], const double *ajac, double *du);
 

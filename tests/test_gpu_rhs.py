@@ -212,3 +212,62 @@ def test_full_size_600x256x128_properties(ws):
     assert np.array_equal(dr, np.roll(du, (3, 5, 7), axis=(1, 2, 3)))
     uni = [np.full((nz, ny, nx), v) for v in (1.1, 0.4, -0.3, 0.2, 1.3)]
     assert np.all(gpu_du_comput(ctx, upload_state(uni)) == 0.0)
+
+
+def _wavy_metrics(nx, ny, nz):
+    """Smooth, non-constant metric terms and Jacobian on the interior (a curvilinear
+    mesh's worth of variation), x fastest: [Akx..Asz, Ajac]."""
+    z, y, x = np.meshgrid(np.arange(nz) / nz, np.arange(ny) / ny, np.arange(nx) / nx, indexing="ij")
+    s = lambda a, b, c: np.sin(2 * np.pi * (a * x + b * y + c * z))
+    A = [1.0 + 0.2 * s(1, 1, 0), 0.1 * s(0, 1, 1), 0.05 * s(1, 0, 1),
+         0.1 * s(1, 1, 1), 1.0 + 0.15 * s(0, 2, 1), -0.08 * s(2, 0, 1),
+         0.06 * s(1, 2, 0), -0.1 * s(0, 1, 2), 1.0 + 0.25 * s(1, 0, 1)]
+    ajac = 1.0 + 0.3 * s(1, 1, 1)
+    return [np.ascontiguousarray(a) for a in A + [ajac]]
+
+
+def test_curvilinear_metrics_from_jacobi_file(orc, ws, tmp_path):
+    """Per-cell metric terms and Jacobian that vary in space, loaded from an
+    OCFD3d-Jacobi.dat file (ws_load_jacobi fills their periodic ghosts): anchor kernels
+    bitwise, tiled FP64 within 1e-12 (every tile shape), mixed within 1e-6 of the oracle
+    given the same metric arrays."""
+    import torch
+    from tests.helpers import ghosted
+    nx, ny, nz = 40, 24, 20
+    prims = random_state(nx, ny, nz, 19, "blobs")
+    met = _wavy_metrics(nx, ny, nz)
+    coords = [np.zeros((nz, ny, nx)) for _ in range(3)]
+    path = str(tmp_path / "OCFD3d-Jacobi.dat").encode()
+    W.check(W.lib().ws_jacobi_write_host(path, nx, ny, nz,
+                                         (ctypes.c_void_p * 13)(*[a.ctypes.data for a in coords + met])))
+    p = orc.params(nx, ny, nz)
+    g = [ghosted(a) for a in met]
+    for a in g:
+        orc.ghost_fill_periodic(p, a)
+    ref = oracle_du(orc, prims, metrics=g[:9], ajac=g[9])
+    tens = [torch.zeros((nz + 8, ny + 8, nx + 8), dtype=torch.float64, device="cuda") for _ in range(10)]
+    mf = W.WsMetricFields()
+    for i in range(9):
+        mf.A[i] = W.field_of(tens[i])
+    mf.ajac = W.field_of(tens[9])
+    ctx = make_ctx(nx, ny, nz)
+    W.check(W.lib().ws_load_jacobi(ctx.handle, path, ctypes.byref(mf), None, W.stream_handle()))
+    torch.cuda.synchronize()
+    for t, a in zip(tens, g):
+        full = t.cpu().numpy()
+        # faces agree with the oracle's periodic fill (edges and corners are never read)
+        assert np.array_equal(full[4:-4, 4:-4, :], a[4:-4, 4:-4, :])
+        assert np.array_equal(full[4:-4, :, 4:-4], a[4:-4, :, 4:-4])
+        assert np.array_equal(full[:, 4:-4, 4:-4], a[:, 4:-4, 4:-4])
+    actx = make_ctx(nx, ny, nz, kernel=W.WS_KERNEL_ANCHOR)
+    assert np.array_equal(gpu_du_comput(actx, upload_state(prims), metrics=mf), ref)
+    for tile in W.TILE_SHAPES:
+        tctx = make_ctx(nx, ny, nz)
+        tctx.set_tile_shape(tile)
+        err = rel_l2(gpu_du_comput(tctx, upload_state(prims), metrics=mf), ref)
+        log(f"  curvilinear tile {tile}: rel L2 {err:.3e}")
+        assert err <= TOL_FP64
+    mctx = make_ctx(nx, ny, nz, precision=W.WS_PREC_MIXED)
+    err = rel_l2(gpu_du_comput(mctx, upload_state(prims), metrics=mf), ref)
+    log(f"  curvilinear mixed: rel L2 {err:.3e}")
+    assert err <= 1e-6

@@ -94,3 +94,25 @@ def test_round_trip_and_grid_checks(tmp_path):
                                   _ptrs(out)) == W.WS_ERR_INVALID
     assert L.ws_restart_read_host(str(tmp_path / "missing.dat").encode(), nx, ny, nz, ctypes.byref(istep),
                                   ctypes.byref(tt), _ptrs(out)) == W.WS_ERR_INVALID
+
+
+def test_jacobi_file_round_trip(tmp_path):
+    """OCFD3d-Jacobi.dat: 13 fields of plane records, no header (the reference reads it
+    from offset 0, src/solver/OCFD_Comput_Jacobian3d.cpp:138-170)."""
+    nx, ny, nz = 5, 4, 3
+    rng = np.random.default_rng(9)
+    f = [np.ascontiguousarray(rng.standard_normal((nz, ny, nx))) for _ in range(13)]
+    path = str(tmp_path / "OCFD3d-Jacobi.dat").encode()
+    L = W.lib()
+    W.check(L.ws_jacobi_write_host(path, nx, ny, nz, (ctypes.c_void_p * 13)(*[a.ctypes.data for a in f])))
+    data = open(path, "rb").read()
+    mark = np.array([nx * ny * 8], dtype=np.int32).tobytes()
+    assert data == b"".join(mark + a[z].tobytes() + mark for a in f for z in range(nz))
+    out = [np.zeros((nz, ny, nx)) for _ in range(13)]
+    W.check(L.ws_jacobi_read_host(path, nx, ny, nz, (ctypes.c_void_p * 13)(*[a.ctypes.data for a in out])))
+    assert all(np.array_equal(a, b) for a, b in zip(out, f))
+    # a restart file is not a Jacobi file of the same grid
+    r5 = str(tmp_path / "r.dat").encode()
+    W.check(L.ws_restart_write_host(r5, nx, ny, nz, 1, 0.0, _ptrs(f[:5])))
+    assert L.ws_jacobi_read_host(r5, nx, ny, nz, (ctypes.c_void_p * 13)(*[a.ctypes.data for a in out])) \
+        == W.WS_ERR_INVALID

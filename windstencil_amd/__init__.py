@@ -53,7 +53,8 @@ class WsConfig(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("ama", ctypes.c_double), ("epsl_sw", ctypes.c_double),
                 ("dt", ctypes.c_double), ("hx", ctypes.c_double), ("hy", ctypes.c_double),
                 ("hz", ctypes.c_double), ("metric", ctypes.c_double * 10), ("precision", _i32),
-                ("kernel", _i32), ("bc_planes", _i32), ("reserved", _i32)]
+                ("kernel", _i32), ("bc_planes", _i32), ("viscous", _i32), ("re", ctypes.c_double),
+                ("pr", ctypes.c_double), ("ref_t", ctypes.c_double)]
 
 
 class WsState(ctypes.Structure):
@@ -85,6 +86,8 @@ _SIGS = {
     "ws_ghost_fill_periodic": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsField), _i32, ctypes.c_void_p]),
     "ws_du_comput": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.POINTER(WsMetricFields),
                                     ctypes.c_void_p]),
+    "ws_viscous_rhs": (ctypes.c_int, [ctypes.c_void_p] + [WsField] * 4 + [ctypes.POINTER(WsMetricFields), WsSoa,
+                                                                      ctypes.c_void_p]),
     "ws_cons_to_pri": (ctypes.c_int, [ctypes.c_void_p, WsSoa] + [WsField] * 6 + [ctypes.c_void_p]),
     "ws_pri_to_cons": (ctypes.c_int, [ctypes.c_void_p, WsSoa] + [WsField] * 5 + [WsJob, ctypes.c_void_p]),
     "ws_bc_planes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p]),
@@ -108,6 +111,10 @@ _SIGS = {
                                        ctypes.c_char_p, ctypes.c_void_p]),
     "ws_load_restart": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_char_p, ctypes.POINTER(_i32),
                                        ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "ws_jacobi_write_host": (ctypes.c_int, [ctypes.c_char_p, _i32, _i32, _i32, ctypes.POINTER(ctypes.c_void_p)]),
+    "ws_jacobi_read_host": (ctypes.c_int, [ctypes.c_char_p, _i32, _i32, _i32, ctypes.POINTER(ctypes.c_void_p)]),
+    "ws_load_jacobi": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(WsMetricFields),
+                                      ctypes.POINTER(WsField), ctypes.c_void_p]),
     "ws_batch_create": (ctypes.c_int, [ctypes.POINTER(WsConfig), _i32, ctypes.POINTER(_i32), ctypes.POINTER(WsState),
                                        ctypes.POINTER(ctypes.c_void_p)]),
     "ws_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -134,6 +141,7 @@ _SIGS = {
     "du_invis_Jacobian3d_x": (None, [WsJob, ctypes.POINTER(WsSoa), ctypes.POINTER(WsSoa),
                                      ctypes.POINTER(ctypes.c_void_p)]),
     "du_invis_Jacobian3d": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "du_vis_Jacobian3d": (ctypes.c_void_p, [ctypes.c_void_p]),
     "du_comput": (None, [ctypes.c_int]),
     "get_duvwT": (None, []),
     "OCFD_time_advance": (None, [ctypes.c_int]),

@@ -236,7 +236,14 @@ int ws_batch_du_comput(ws_batch *b, hipStream_t s)
         int r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
         if (r) return r;
     }
-    return batch_rhs(b, s);
+    int r = batch_rhs(b, s);
+    if (r || !b->base.viscous) return r;
+    // viscous terms, block by block (memory-bound kernels; each block owns its scratch)
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        const ws_state &st = b->st[i];
+        if ((r = launch_viscous(*b->ctx[i], st.u, st.v, st.w, st.T, nullptr, st.du, s))) return r;
+    }
+    return WS_OK;
 }
 
 int ws_batch_cfl_dt(ws_batch *b, double cfl, double *dt_dev, hipStream_t s)

@@ -133,6 +133,10 @@ int validate_config(const ws_config *cfg)
         return set_error(WS_ERR_INVALID, "unknown precision");
     if (cfg->kernel != WS_KERNEL_TILED && cfg->kernel != WS_KERNEL_ANCHOR)
         return set_error(WS_ERR_INVALID, "unknown kernel family");
+    if (cfg->viscous != 0 && cfg->viscous != 6 && cfg->viscous != 8)
+        return set_error(WS_ERR_INVALID, "viscous must be 0 (off), 6 or 8 (central-difference order)");
+    if (cfg->viscous && !(cfg->re > 0.0 && cfg->pr > 0.0 && cfg->ref_t > 0.0))
+        return set_error(WS_ERR_INVALID, "viscous terms need positive Re, Pr and Ref_T");
     return WS_OK;
 }
 
@@ -238,6 +242,10 @@ void ws_config_default(ws_config *cfg, int32_t nx, int32_t ny, int32_t nz)
     cfg->precision = WS_PREC_FP64;
     cfg->kernel = WS_KERNEL_TILED;
     cfg->bc_planes = 1;
+    cfg->viscous = 0;
+    cfg->re = 1000.0;
+    cfg->pr = 0.72;
+    cfg->ref_t = 288.15;
 }
 
 int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
@@ -271,6 +279,10 @@ int ws_ctx_destroy(ws_ctx *c)
     if (!c) return WS_OK;
     if (c->capture_stream) hipStreamDestroy(c->capture_stream);
     if (c->cfl_partial) hipFree(c->cfl_partial);
+    if (c->vis_scratch) {
+        hipDeviceSynchronize();
+        hipFree(c->vis_scratch);
+    }
     delete c;
     return WS_OK;
 }
@@ -403,7 +415,20 @@ int ws_du_comput(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics,
     const ws_field g[5] = {st->d, st->u, st->v, st->w, st->T};
     if ((r = ws_ghost_fill_periodic(c, g, 5, s))) return r;
     if ((r = launch_sound_speed(*c, st->T, st->cc, ghosted_job(c), s))) return r;
-    return ws_inviscid_rhs(c, st->d, st->u, st->v, st->w, st->cc, metrics, st->du, interior_job(c), 0, s);
+    if ((r = ws_inviscid_rhs(c, st->d, st->u, st->v, st->w, st->cc, metrics, st->du, interior_job(c), 0, s))) return r;
+    if (c->cfg.viscous) return launch_viscous(*c, st->u, st->v, st->w, st->T, metrics, st->du, s);
+    return WS_OK;
+}
+
+int ws_viscous_rhs(ws_ctx *c, ws_field u, ws_field v, ws_field w, ws_field T, const ws_metric_fields *metrics,
+                   ws_soa du, hipStream_t s)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    int r;
+    if ((r = check_field(c, u, "u")) || (r = check_field(c, v, "v")) || (r = check_field(c, w, "w")) ||
+        (r = check_field(c, T, "T")) || (r = check_soa(c, du, "du")) || (r = check_metrics(c, metrics)))
+        return r;
+    return launch_viscous(*c, u, v, w, T, metrics, du, s);
 }
 
 int ws_cons_to_pri(ws_ctx *c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T, ws_field P,

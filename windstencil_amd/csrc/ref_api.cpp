@@ -100,6 +100,15 @@ void *du_invis_Jacobian3d(void *pthread_id)
     return nullptr;
 }
 
+void *du_vis_Jacobian3d(void *pthread_id)
+{
+    (void)pthread_id;
+    if (!bound(__func__)) return nullptr;
+    const ws_state *s = g_bound.st;
+    fail(__func__, ws_viscous_rhs(g_bound.ctx, s->u, s->v, s->w, s->T, g_bound.metrics, s->du, g_bound.stream));
+    return nullptr;
+}
+
 void du_comput(int KRK)
 {
     (void)KRK;
@@ -107,6 +116,10 @@ void du_comput(int KRK)
     // the reference zeroes du first (OCFD_Stream.cpp:35); the RHS here overwrites du,
     // which leaves the same result without the extra pass
     du_invis_Jacobian3d(nullptr);
+    if (g_last_status) return;
+    ws_config c;
+    ws_ctx_get_config(g_bound.ctx, &c);
+    if (c.viscous) du_vis_Jacobian3d(nullptr);
 }
 
 void get_duvwT(void)

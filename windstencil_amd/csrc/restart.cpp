@@ -1,4 +1,5 @@
-// restart.cpp -- restart files in the reference's format (OCFD%08d.dat).
+// restart.cpp -- restart files in the reference's format (OCFD%08d.dat) and its
+// metric file (OCFD3d-Jacobi.dat).
 //
 // Layout (src/io/OCFD_IO.c:153-226 writes it, :24-90 reads it; the per-plane records
 // are written by src/io/OCFD_IO_mpi.c:605-701 and read by :563-604), native byte order:
@@ -6,6 +7,9 @@
 //   then rho, u, v, w, T, each as NZ records, one per z plane (z ascending):
 //                   int32 NX*NY*8 | NX*NY float64 (x fastest, then y) | int32 NX*NY*8
 // i.e. Fortran sequential unformatted records.  Only the interior is stored.
+// OCFD3d-Jacobi.dat (read by Init_Jacobian3d, src/solver/OCFD_Comput_Jacobian3d.cpp:138-205)
+// has no header: x, y, z, Akx, Aky, Akz, Aix, Aiy, Aiz, Asx, Asy, Asz, Ajac as NZ plane
+// records each.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -27,6 +31,37 @@ struct File {
 bool put_i32(FILE *f, int32_t v) { return std::fwrite(&v, sizeof(v), 1, f) == 1; }
 
 bool get_i32(FILE *f, int32_t *v) { return std::fread(v, sizeof(*v), 1, f) == 1; }
+
+// nf fields of nz plane records each
+bool write_planes(FILE *f, int32_t nx, int32_t ny, int32_t nz, int nf, const double *const *fields)
+{
+    const size_t plane = (size_t)nx * ny;
+    const int32_t rec = (int32_t)(plane * sizeof(double));
+    bool ok = true;
+    for (int m = 0; m < nf && ok; ++m)
+        for (int z = 0; z < nz && ok; ++z)
+            ok = put_i32(f, rec) && std::fwrite(fields[m] + (size_t)z * plane, sizeof(double), plane, f) == plane &&
+                 put_i32(f, rec);
+    return ok;
+}
+
+int read_planes(FILE *f, int32_t nx, int32_t ny, int32_t nz, int nf, double *const *fields)
+{
+    const size_t plane = (size_t)nx * ny;
+    const int32_t rec = (int32_t)(plane * sizeof(double));
+    int32_t a = 0, b = 0;
+    for (int m = 0; m < nf; ++m)
+        for (int z = 0; z < nz; ++z) {
+            if (!get_i32(f, &a) || a != rec)
+                return set_error(WS_ERR_INVALID, "plane record does not match nx*ny of this grid");
+            if (std::fread(fields[m] + (size_t)z * plane, sizeof(double), plane, f) != plane || !get_i32(f, &b) ||
+                b != rec)
+                return set_error(WS_ERR_INVALID, "truncated plane record");
+        }
+    // the file must end here: a longer file belongs to a larger grid
+    if (std::fgetc(f) != EOF) return set_error(WS_ERR_INVALID, "file longer than this grid's data");
+    return WS_OK;
+}
 
 int check_extents(int32_t nx, int32_t ny, int32_t nz)
 {
@@ -88,12 +123,7 @@ int ws_restart_write_host(const char *path, int32_t nx, int32_t ny, int32_t nz, 
     const int32_t hdr = (int32_t)(sizeof(int32_t) + sizeof(double));
     bool ok = put_i32(fh.f, hdr) && put_i32(fh.f, istep) && std::fwrite(&tt, sizeof(tt), 1, fh.f) == 1 &&
               put_i32(fh.f, hdr);
-    const size_t plane = (size_t)nx * ny;
-    const int32_t rec = (int32_t)(plane * sizeof(double));
-    for (int m = 0; m < 5 && ok; ++m)
-        for (int z = 0; z < nz && ok; ++z)
-            ok = put_i32(fh.f, rec) && std::fwrite(fields[m] + (size_t)z * plane, sizeof(double), plane, fh.f) == plane &&
-                 put_i32(fh.f, rec);
+    ok = ok && write_planes(fh.f, nx, ny, nz, 5, fields);
     if (!ok) return set_error(WS_ERR_INVALID, std::string("restart write: short write to ") + path);
     if (std::fclose(fh.f) != 0) { fh.f = nullptr; return set_error(WS_ERR_INVALID, "restart write: close failed"); }
     fh.f = nullptr;
@@ -116,18 +146,8 @@ int ws_restart_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, i
     if (!get_i32(fh.f, &a) || !get_i32(fh.f, istep) || std::fread(tt, sizeof(*tt), 1, fh.f) != 1 ||
         !get_i32(fh.f, &b) || a != hdr || b != hdr)
         return set_error(WS_ERR_INVALID, "restart read: bad header record");
-    const size_t plane = (size_t)nx * ny;
-    const int32_t rec = (int32_t)(plane * sizeof(double));
-    for (int m = 0; m < 5; ++m)
-        for (int z = 0; z < nz; ++z) {
-            if (!get_i32(fh.f, &a) || a != rec)
-                return set_error(WS_ERR_INVALID, "restart read: plane record does not match nx*ny of this grid");
-            if (std::fread(fields[m] + (size_t)z * plane, sizeof(double), plane, fh.f) != plane || !get_i32(fh.f, &b) ||
-                b != rec)
-                return set_error(WS_ERR_INVALID, "restart read: truncated plane record");
-        }
-    // the file must end here: a longer file belongs to a larger grid
-    if (std::fgetc(fh.f) != EOF) return set_error(WS_ERR_INVALID, "restart read: file longer than this grid's data");
+    if ((r = read_planes(fh.f, nx, ny, nz, 5, fields)))
+        return set_error(r, std::string("restart read: ") + ws_last_error());
     return WS_OK;
 }
 
@@ -165,6 +185,66 @@ int ws_load_restart(ws_ctx *c, ws_state *st, const char *path, int32_t *istep, d
         if (!fl[m].ptr || fl[m].pitch < (uint32_t)(nx + 2 * LAP)) return set_error(WS_ERR_INVALID, "restart load: bad field");
         if ((r = host_to_field(p[m], nx, ny, nz, fl[m]))) return r;
     }
+    return WS_OK;
+}
+
+int ws_jacobi_write_host(const char *path, int32_t nx, int32_t ny, int32_t nz, const double *const *fields)
+{
+    if (!path || !fields) return set_error(WS_ERR_INVALID, "Jacobi write: null argument");
+    int r = check_extents(nx, ny, nz);
+    if (r) return r;
+    for (int m = 0; m < 13; ++m)
+        if (!fields[m]) return set_error(WS_ERR_INVALID, "Jacobi write: null field");
+    File fh;
+    fh.f = std::fopen(path, "wb");
+    if (!fh.f) return set_error(WS_ERR_INVALID, std::string("Jacobi write: cannot open ") + path);
+    if (!write_planes(fh.f, nx, ny, nz, 13, fields))
+        return set_error(WS_ERR_INVALID, std::string("Jacobi write: short write to ") + path);
+    if (std::fclose(fh.f) != 0) { fh.f = nullptr; return set_error(WS_ERR_INVALID, "Jacobi write: close failed"); }
+    fh.f = nullptr;
+    return WS_OK;
+}
+
+int ws_jacobi_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, double *const *fields)
+{
+    if (!path || !fields) return set_error(WS_ERR_INVALID, "Jacobi read: null argument");
+    int r = check_extents(nx, ny, nz);
+    if (r) return r;
+    for (int m = 0; m < 13; ++m)
+        if (!fields[m]) return set_error(WS_ERR_INVALID, "Jacobi read: null field");
+    File fh;
+    fh.f = std::fopen(path, "rb");
+    if (!fh.f) return set_error(WS_ERR_INVALID, std::string("Jacobi read: cannot open ") + path);
+    if ((r = read_planes(fh.f, nx, ny, nz, 13, fields)))
+        return set_error(r, std::string("Jacobi read: ") + ws_last_error());
+    return WS_OK;
+}
+
+int ws_load_jacobi(ws_ctx *c, const char *path, const ws_metric_fields *mf, const ws_field *coords,
+                   hipStream_t s)
+{
+    if (!c || !path || !mf) return set_error(WS_ERR_INVALID, "Jacobi load: null argument");
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    ws_field dst[13];
+    for (int m = 0; m < 3; ++m) dst[m] = coords ? coords[m] : ws_field{};
+    for (int m = 0; m < 9; ++m) dst[3 + m] = mf->A[m];
+    dst[12] = mf->ajac;
+    for (int m = 0; m < 13; ++m)
+        if ((m >= 3 || coords) && (!dst[m].ptr || dst[m].pitch < (uint32_t)(nx + 2 * LAP)))
+            return set_error(WS_ERR_INVALID, "Jacobi load: bad destination field");
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<double> host(13 * n);
+    double *p[13];
+    for (int m = 0; m < 13; ++m) p[m] = host.data() + m * n;
+    int r = ws_jacobi_read_host(path, nx, ny, nz, p);
+    if (r) return r;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(e, "Jacobi load: stream sync");
+    for (int m = coords ? 0 : 3; m < 13; ++m)
+        if ((r = host_to_field(p[m], nx, ny, nz, dst[m]))) return r;
+    // periodic face ghosts of the metric terms (the reference's exchange_boundary_xyz)
+    if ((r = launch_ghost_fill(*c, dst + 3, 10, s))) return r;
+    if (coords && (r = launch_ghost_fill(*c, dst, 3, s))) return r;
     return WS_OK;
 }
 

@@ -46,6 +46,9 @@ struct Ctx {
     int tile_shape = WS_TILE_30x16_K2;
     // how the tiled kernels walk their tiles (WS_SCHED_*)
     int schedule = WS_SCHED_GRID;
+    // viscous terms: twelve ghosted flux fields, allocated at the first use
+    double *vis_scratch = nullptr;
+    size_t vis_scratch_elems = 0;
 };
 
 KParams make_kparams(const ws_config &cfg);
@@ -94,6 +97,10 @@ int check_state(const ws_ctx *c, const ws_state *st);
 int prepare_rhs_inputs(ws_ctx *c, const ws_state *st, hipStream_t s);
 // the RHS over the interior with the context's kernel and precision (overwrites du)
 int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s);
+
+// viscous terms added to du[1..4] (k_viscous.hip); allocates the context's scratch once
+int launch_viscous(Ctx &c, ws_field u, ws_field v, ws_field w, ws_field T, const ws_metric_fields *m, ws_soa du,
+                   hipStream_t s);
 
 int launch_sound_speed(const Ctx &c, ws_field T, ws_field cc, ws_job job, hipStream_t s);
 int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s);
