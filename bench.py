@@ -29,6 +29,9 @@ METRIC = "inviscid RHS Mcell-updates/s, FP64 vs mixed at 600x256x128; % attainab
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_HBM_GBS = 8000.0
 ALGO_BYTES_PER_CELL = 80     # 5 FP64 state in + 5 FP64 RHS out
+# viscous kernels: u, v, w, T in (32 B) and twelve flux fields out (96 B); the twelve
+# fluxes in again (96 B) and du[1..4] read and written (64 B)
+VISCOUS_BYTES_PER_CELL = 288
 
 
 def parse():
@@ -41,6 +44,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline leg")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-viscous", action="store_true", help="skip the Navier-Stokes (viscous) leg")
     ap.add_argument("--no-second", action="store_true",
                     help="skip timing the other precision (mixed when --precision fp64)")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape (default: autotune)")
@@ -348,6 +352,48 @@ def main():
         second["speedup_mixed_over_fp64"] = round(fp64_ms / mixed_ms, 4)
         del du2
 
+    viscous = None
+    if not args.no_viscous:
+        # the Navier-Stokes RHS: the same inviscid RHS plus the viscous terms (CD6,
+        # Re 1000), timed on the same state; the viscous kernels are HBM-bound
+        vctx = W.Context(W.default_config(nx, ny, nz, viscous=6))
+        W.check(L.ws_ctx_set_tile_shape(vctx.handle, int(shape.split("/")[0])))
+        W.check(L.ws_ctx_set_schedule(vctx.handle, int(shape.split("/")[1])))
+
+        def ns_rhs(ev=None):
+            W.check(L.ws_ghost_fill_periodic(vctx.handle, fl, 5, sh))
+            W.check(L.ws_sound_speed(vctx.handle, st.ws.T, st.ws.cc, gjob, sh))
+            W.check(L.ws_inviscid_rhs(vctx.handle, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.cc, None, st.ws.du,
+                                      ijob, 0, sh))
+            if ev is not None:
+                ev[0].record(stream)
+            W.check(L.ws_viscous_rhs(vctx.handle, st.ws.u, st.ws.v, st.ws.w, st.ws.T, None, st.ws.du, sh))
+            if ev is not None:
+                ev[1].record(stream)
+
+        for _ in range(args.warmup):
+            ns_rhs()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ns_rhs()
+        torch.cuda.synchronize()
+        ns_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
+        for k in range(args.steps):
+            ns_rhs(evs[k])
+        torch.cuda.synchronize()
+        vis_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
+        vis_bytes = VISCOUS_BYTES_PER_CELL * N
+        viscous = {"workload": "inviscid + viscous (CD6, Re 1000, Sutherland) RHS, FP64",
+                   "value": round(N / (ns_ms * 1e-3) / 1e6, 2), "unit": "Mcell-updates/s",
+                   "ms_per_step": round(ns_ms, 4), "viscous_kernels_ms": round(vis_ms, 4),
+                   "viscous_roofline": {"bound": "hbm", "achieved": round(vis_bytes / (vis_ms * 1e-3) / 1e9, 1),
+                                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                        "frac": round(vis_bytes / (vis_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                        "bytes_per_cell": VISCOUS_BYTES_PER_CELL}}
+        del vctx
+
     parity = None
     if not args.no_parity:
         # fast path against the reference-order anchor kernels on the same state
@@ -391,6 +437,7 @@ def main():
         "autotune_key": "tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
         "parity_rel_l2_vs_anchor": parity,
         "other_precision": second,
+        "navier_stokes": viscous,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prims, nx, ny, nz, args.cpu_seconds)
