@@ -24,7 +24,8 @@ class OrcParams(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("ama", ctypes.c_double), ("epsl_sw", ctypes.c_double),
                 ("hx", ctypes.c_double), ("hy", ctypes.c_double), ("hz", ctypes.c_double),
                 ("bc_planes", ctypes.c_int), ("nthreads", ctypes.c_int), ("viscous", ctypes.c_int),
-                ("re", ctypes.c_double), ("pr", ctypes.c_double), ("ref_t", ctypes.c_double)]
+                ("re", ctypes.c_double), ("pr", ctypes.c_double), ("ref_t", ctypes.c_double),
+                ("periodic", ctypes.c_int * 3), ("d0_bound", ctypes.c_int * 6), ("non_ref", ctypes.c_int * 6)]
 
 
 def build():
@@ -45,6 +46,9 @@ def lib():
         L.orc_weno7_symbo_p_parts.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_split_sw.argtypes = [ctypes.c_double] * 10 + [_dp, _dp]
         L.orc_cfl_dt.restype = ctypes.c_double
+        for fn in (L.orc_closure_p, L.orc_closure_m):
+            fn.restype = ctypes.c_int
+            fn.argtypes = [_dp, ctypes.c_long, ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_cd6.restype = ctypes.c_double
         L.orc_cd6.argtypes = [_dp]
         L.orc_cd8.restype = ctypes.c_double
@@ -59,10 +63,11 @@ def _ptr(a):
 
 
 def params(nx, ny, nz, gamma=1.4, ama=1.0, epsl_sw=1e-3, h=None, bc_planes=1, nthreads=0, viscous=0,
-           re=1000.0, pr=0.72, ref_t=288.15):
+           re=1000.0, pr=0.72, ref_t=288.15, periodic=(1, 1, 1), d0_bound=(0,) * 6, non_ref=(0,) * 6):
     if h is None:
         h = (1.0 / (nx - 1.0), 1.0 / (ny - 1.0), 1.0 / (nz - 1.0))
-    return OrcParams(nx, ny, nz, gamma, ama, epsl_sw, h[0], h[1], h[2], bc_planes, nthreads, viscous, re, pr, ref_t)
+    return OrcParams(nx, ny, nz, gamma, ama, epsl_sw, h[0], h[1], h[2], bc_planes, nthreads, viscous, re, pr, ref_t,
+                     (ctypes.c_int * 3)(*periodic), (ctypes.c_int * 6)(*d0_bound), (ctypes.c_int * 6)(*non_ref))
 
 
 def weno_p(s):
@@ -155,6 +160,17 @@ def cd(s):
     """Central first-derivative stencil (CD6 for 7 points, CD8 for 9), before 1/h."""
     s = np.ascontiguousarray(s, dtype=np.float64)
     return lib().orc_cd6(_ptr(s)) if len(s) == 7 else lib().orc_cd8(_ptr(s))
+
+
+def closure(side, line, f, lo=1, hi=1):
+    """Closure face value of face f on a line given as an array of n+8 values (4 ghosts
+    each side), or None if no closure covers the face.  side: 'p' or 'm'."""
+    line = np.ascontiguousarray(line, dtype=np.float64)
+    n = len(line) - 8
+    out = ctypes.c_double()
+    fn = lib().orc_closure_p if side == "p" else lib().orc_closure_m
+    ok = fn(ctypes.cast(line.ctypes.data + 4 * 8, _dp), n, f, lo, hi, ctypes.byref(out))
+    return out.value if ok else None
 
 
 def amu(p, T, re, ref_t):

@@ -129,24 +129,27 @@ void orc_ghost_fill_periodic(const orc_params *p, double *fld)
 {
     const long nx = p->nx, ny = p->ny, nz = p->nz;
     long i, j, k;
-    for (k = 0; k < nz; k++)
-        for (j = 0; j < ny; j++)
-            for (i = 0; i < LAP; i++) {
-                fld[G(p, nx + i, j, k)] = fld[G(p, i, j, k)];          /* right ghost <- left interior */
-                fld[G(p, i - LAP, j, k)] = fld[G(p, nx - LAP + i, j, k)]; /* left ghost <- right interior */
-            }
-    for (k = 0; k < nz; k++)
-        for (j = 0; j < LAP; j++)
-            for (i = 0; i < nx; i++) {
-                fld[G(p, i, ny + j, k)] = fld[G(p, i, j, k)];
-                fld[G(p, i, j - LAP, k)] = fld[G(p, i, ny - LAP + j, k)];
-            }
-    for (k = 0; k < LAP; k++)
-        for (j = 0; j < ny; j++)
-            for (i = 0; i < nx; i++) {
-                fld[G(p, i, j, nz + k)] = fld[G(p, i, j, k)];
-                fld[G(p, i, j, k - LAP)] = fld[G(p, i, j, nz - LAP + k)];
-            }
+    if (p->periodic[0])
+        for (k = 0; k < nz; k++)
+            for (j = 0; j < ny; j++)
+                for (i = 0; i < LAP; i++) {
+                    fld[G(p, nx + i, j, k)] = fld[G(p, i, j, k)];          /* right ghost <- left interior */
+                    fld[G(p, i - LAP, j, k)] = fld[G(p, nx - LAP + i, j, k)]; /* left ghost <- right interior */
+                }
+    if (p->periodic[1])
+        for (k = 0; k < nz; k++)
+            for (j = 0; j < LAP; j++)
+                for (i = 0; i < nx; i++) {
+                    fld[G(p, i, ny + j, k)] = fld[G(p, i, j, k)];
+                    fld[G(p, i, j - LAP, k)] = fld[G(p, i, ny - LAP + j, k)];
+                }
+    if (p->periodic[2])
+        for (k = 0; k < LAP; k++)
+            for (j = 0; j < ny; j++)
+                for (i = 0; i < nx; i++) {
+                    fld[G(p, i, j, nz + k)] = fld[G(p, i, j, k)];
+                    fld[G(p, i, j, k - LAP)] = fld[G(p, i, j, nz - LAP + k)];
+                }
 }
 
 /* cc over the whole ghosted box (du_invis_Jacobian3d_init launches the
@@ -160,13 +163,199 @@ void orc_sound_speed(const orc_params *p, const double *T, double *cc)
 
 /* ---------------------------------------------------------------- RHS */
 
+/* ---------------------------------------------------------------- boundary closures */
+
+/* sign(1, x) and minmod2 (src/kernels/OCFD_Schemes.cpp:822-835) */
+static double bs_sign1(double x) { return x >= 0 ? 1.0 : -1.0; }
+static double bs_minmod2(double a, double b)
+{
+    return 0.5 * (bs_sign1(a) + bs_sign1(b)) * fmin(fabs(a), fabs(b));
+}
+
+/* Reduced WENO5 reconstructions on a 5-point window s[0..4] (no linear weights), used by
+ * the one-sided closures.  OCFD_weno5_kernel_P (OCFD_Schemes.cpp:1110-1141) builds a0 and a1
+ * from S1 and S2 before they are assigned (both still 0) -- reproduced as written. */
+static double w5_p(const double *s)
+{
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0, tmp, a0, a1, a2, q03, q13, q23;
+    const double ep = 1e-6;
+    tmp = s[2] - 2.0 * s[3] + s[4]; S0 += 13 * tmp * tmp;
+    tmp = 3.0 * s[2] - 4.0 * s[3] + s[4]; S0 += 3 * tmp * tmp;
+    a0 = ((12.0 * ep + S1) * (12.0 * ep + S1)) * ((12.0 * ep + S2) * (12.0 * ep + S2));
+    q03 = (2.0 * s[2] + 5.0 * s[3] - s[4]);
+    tmp = s[1] - 2.0 * s[2] + s[3]; S1 += 13 * tmp * tmp;
+    tmp = s[1] - s[3]; S1 += 3 * tmp * tmp;
+    a1 = ((12.0 * ep + S0) * (12.0 * ep + S0)) * ((12.0 * ep + S2) * (12.0 * ep + S2));
+    q13 = (-s[1] + 5.0 * s[2] + 2.0 * s[3]);
+    tmp = s[0] - 2.0 * s[1] + s[2]; S2 += 13 * tmp * tmp;
+    tmp = s[0] - 4.0 * s[1] + 3.0 * s[2]; S2 += 3 * tmp * tmp;
+    a2 = ((12.0 * ep + S0) * (12.0 * ep + S0)) * ((12.0 * ep + S1) * (12.0 * ep + S1));
+    q23 = (2.0 * s[0] - 7.0 * s[1] + 11.0 * s[2]);
+    return (a0 * q03 + a1 * q13 + a2 * q23) / (6.0 * (a0 + a1 + a2));
+}
+
+/* OCFD_weno5_kernel_M, OCFD_Schemes.cpp:1170-1201 */
+static double w5_m(const double *s)
+{
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0, tmp, a0, a1, a2, q03, q13, q23;
+    const double ep = 1e-6;
+    tmp = s[2] - 2.0 * s[1] + s[0]; S0 += 13 * tmp * tmp;
+    tmp = 3.0 * s[2] - 4.0 * s[1] + s[0]; S0 += 3 * tmp * tmp;
+    a0 = 1.0 / ((12.0 * ep + S0) * (12.0 * ep + S0));
+    q03 = (2.0 * s[2] + 5.0 * s[1] - s[0]);
+    tmp = s[3] - 2.0 * s[2] + s[1]; S1 += 13 * tmp * tmp;
+    tmp = s[3] - s[1]; S1 += 3 * tmp * tmp;
+    a1 = 1.0 / ((12.0 * ep + S1) * (12.0 * ep + S1));
+    q13 = (-s[3] + 5.0 * s[2] + 2.0 * s[1]);
+    tmp = s[4] - 2.0 * s[3] + s[2]; S2 += 13 * tmp * tmp;
+    tmp = s[4] - 4.0 * s[3] + 3.0 * s[2]; S2 += 3 * tmp * tmp;
+    a2 = 1.0 / ((12.0 * ep + S2) * (12.0 * ep + S2));
+    q23 = (2.0 * s[4] - 7.0 * s[3] + 11.0 * s[2]);
+    return (a0 * q03 + a1 * q13 + a2 * q23) / (6.0 * (a0 + a1 + a2));
+}
+
+/* one-candidate and two-candidate closures, src/boundary/OCFD_bound_Scheme.cpp:540-719 */
+static double w5_p_lift(const double *s)
+{
+    double S0 = 0.0, tmp, a0, q03;
+    const double ep = 1e-6;
+    tmp = s[2] - 2.0 * s[3] + s[4]; S0 += 13 * tmp * tmp;
+    tmp = 3.0 * s[2] - 4.0 * s[3] + s[4]; S0 += 3 * tmp * tmp;
+    a0 = 1.0 / ((12.0 * ep + S0) * (12.0 * ep + S0));
+    q03 = (2.0 * s[2] + 5.0 * s[3] - s[4]);
+    return a0 * q03 / (6.0 * a0);
+}
+
+static double w5_p_lift_plus(const double *s)
+{
+    double S0 = 0.0, S1 = 0.0, tmp, a0, a1, q03, q13;
+    const double ep = 1e-6;
+    tmp = s[2] - 2.0 * s[3] + s[4]; S0 += 13 * tmp * tmp;
+    tmp = 3.0 * s[2] - 4.0 * s[3] + s[4]; S0 += 3 * tmp * tmp;
+    a0 = 1.0 / ((12.0 * ep + S0) * (12.0 * ep + S0));
+    q03 = (2.0 * s[2] + 5.0 * s[3] - s[4]);
+    tmp = s[1] - 2.0 * s[2] + s[3]; S1 += 13 * tmp * tmp;
+    tmp = s[1] - s[3]; S1 += 3 * tmp * tmp;
+    a1 = 1.0 / ((12.0 * ep + S1) * (12.0 * ep + S1));
+    q13 = (-s[1] + 5.0 * s[2] + 2.0 * s[3]);
+    return (a0 * q03 + a1 * q13) / (6.0 * (a0 + a1));
+}
+
+static double w5_p_right_plus(const double *s)
+{
+    double S1 = 0.0, S2 = 0.0, tmp, a1, a2, q13, q23;
+    const double ep = 1e-6;
+    tmp = s[1] - 2.0 * s[2] + s[3]; S1 += 13 * tmp * tmp;
+    tmp = s[1] - s[3]; S1 += 3 * tmp * tmp;
+    a1 = 1.0 / ((12.0 * ep + S1) * (12.0 * ep + S1));
+    q13 = (-s[1] + 5.0 * s[2] + 2.0 * s[3]);
+    tmp = s[0] - 2.0 * s[1] + s[2]; S2 += 13 * tmp * tmp;
+    tmp = s[0] - 4.0 * s[1] + 3.0 * s[2]; S2 += 3 * tmp * tmp;
+    a2 = 1.0 / ((12.0 * ep + S2) * (12.0 * ep + S2));
+    q23 = (2.0 * s[0] - 7.0 * s[1] + 11.0 * s[2]);
+    return (a1 * q13 + a2 * q23) / (6.0 * (a1 + a2));
+}
+
+static double w5_m_lift(const double *s)
+{
+    double S2 = 0.0, tmp, a2, q23;
+    const double ep = 1e-6;
+    tmp = s[4] - 2.0 * s[3] + s[2]; S2 += 13 * tmp * tmp;
+    tmp = s[4] - 4.0 * s[3] + 3.0 * s[2]; S2 += 3 * tmp * tmp;
+    a2 = 1.0 / ((12.0 * ep + S2) * (12.0 * ep + S2));
+    q23 = (2.0 * s[4] - 7.0 * s[3] + 11.0 * s[2]);
+    return a2 * q23 / (6.0 * a2);
+}
+
+static double w5_m_lift_plus(const double *s)
+{
+    double S1 = 0.0, S2 = 0.0, tmp, a1, a2, q13, q23;
+    const double ep = 1e-6;
+    tmp = s[3] - 2.0 * s[2] + s[1]; S1 += 13 * tmp * tmp;
+    tmp = s[3] - s[1]; S1 += 3 * tmp * tmp;
+    a1 = 1.0 / ((12.0 * ep + S1) * (12.0 * ep + S1));
+    q13 = (-s[3] + 5.0 * s[2] + 2.0 * s[1]);
+    tmp = s[4] - 2.0 * s[3] + s[2]; S2 += 13 * tmp * tmp;
+    tmp = s[4] - 4.0 * s[3] + 3.0 * s[2]; S2 += 3 * tmp * tmp;
+    a2 = 1.0 / ((12.0 * ep + S2) * (12.0 * ep + S2));
+    q23 = (2.0 * s[4] - 7.0 * s[3] + 11.0 * s[2]);
+    return (a1 * q13 + a2 * q23) / (6.0 * (a1 + a2));
+}
+
+static double w5_m_right_plus(const double *s)
+{
+    double S0 = 0.0, S1 = 0.0, tmp, a0, a1, q03, q13;
+    const double ep = 1e-6;
+    tmp = s[2] - 2.0 * s[1] + s[0]; S0 += 13 * tmp * tmp;
+    tmp = 3.0 * s[2] - 4.0 * s[1] + s[0]; S0 += 3 * tmp * tmp;
+    a0 = 1.0 / ((12.0 * ep + S0) * (12.0 * ep + S0));
+    q03 = (2.0 * s[2] + 5.0 * s[1] - s[0]);
+    tmp = s[3] - 2.0 * s[2] + s[1]; S1 += 13 * tmp * tmp;
+    tmp = s[3] - s[1]; S1 += 3 * tmp * tmp;
+    a1 = 1.0 / ((12.0 * ep + S1) * (12.0 * ep + S1));
+    q13 = (-s[3] + 5.0 * s[2] + 2.0 * s[1]);
+    return (a0 * q03 + a1 * q13) / (6.0 * (a0 + a1));
+}
+
+/* P face f (between cells f-1 and f) of a line with n cells and one-sided closures at the
+ * low (lo) / high (hi) end, OCFD_bound_scheme_kernel_p (OCFD_bound_Scheme.cpp:722-807): the
+ * kernel's thread c = f holds cell f-1 and the window s(f-4 .. f+3); the 5-point closures
+ * use cells f-3 .. f+1.  s points at cell 0.  Returns 1 if a closure applies. */
+static int closure_p(const double *s, long n, long f, int lo, int hi, double *out)
+{
+    if (lo && f <= 3) {
+        if (f == 0) *out = s[0];
+        else if (f == 1) *out = w5_p_lift(s + f - 3);
+        else if (f == 2) *out = w5_p_lift_plus(s + f - 3);
+        else *out = w5_p(s + f - 3);
+        return 1;
+    }
+    if (hi && f >= n - 2) {
+        if (f == n) *out = s[n - 1] + 0.5 * bs_minmod2(s[n - 1] - s[n - 2], s[n - 1] - s[n - 2]);
+        else if (f == n - 1) *out = w5_p_right_plus(s + f - 3);
+        else *out = w5_p(s + f - 3);
+        return 1;
+    }
+    return 0;
+}
+
+/* M face f, OCFD_bound_scheme_kernel_m (OCFD_bound_Scheme.cpp:808-893): thread c = f holds
+ * cell f and the window s(f-4 .. f+3); the 5-point closures use cells f-2 .. f+2. */
+static int closure_m(const double *s, long n, long f, int lo, int hi, double *out)
+{
+    if (lo && f <= 3) {
+        if (f == 0) *out = w5_m_lift(s + f - 2);
+        else if (f == 1) *out = w5_m_lift_plus(s + f - 2);
+        else *out = w5_m(s + f - 2);
+        return 1;
+    }
+    if (hi && f >= n - 3) {
+        if (f == n) *out = s[n - 1];
+        else if (f == n - 1) *out = s[n - 1] - 0.5 * bs_minmod2(s[n - 1] - s[n - 2], s[n - 1] - s[n - 2]);
+        else if (f == n - 2) *out = w5_m_right_plus(s + f - 2);
+        else *out = w5_m(s + f - 2);
+        return 1;
+    }
+    return 0;
+}
+
+int orc_closure_p(const double *s, long n, long f, int lo, int hi, double *out)
+{
+    return closure_p(s, n, f, lo, hi, out);
+}
+
+int orc_closure_m(const double *s, long n, long f, int lo, int hi, double *out)
+{
+    return closure_m(s, n, f, lo, hi, out);
+}
+
 /* One direction of the RHS on one line: split at the n+2*LAP points, then the
  * P faces for all five variables (OCFD_weno7_SYMBO_P_kernel + put_du_p_kernel),
  * then the M faces (OCFD_weno7_SYMBO_M_kernel + put_du_m_kernel).
  * Face f (f = 0..n) sits between cells f-1 and f and reconstructs from the
  * split values at f-4 .. f+3.  prim[]/metric pointers are pre-offset per
  * point by the caller through the `at` index array. */
-static void rhs_line(const orc_params *p, long n, const size_t *at, const size_t *du_at, double h,
+static void rhs_line(const orc_params *p, int dir, long n, const size_t *at, const size_t *du_at, double h,
                      const double *d, const double *u, const double *v, const double *w, const double *cc,
                      const double *A1, const double *A2, const double *A3, const double *ajac,
                      double *du, double *buf)
@@ -188,17 +377,25 @@ static void rhs_line(const orc_params *p, long n, const size_t *at, const size_t
                           p->gamma, p->epsl_sw, C1, C3, fp5, fm5);
         for (m = 0; m < 5; m++) { fpv[m * np + q] = fp5[m]; fmv[m * np + q] = fm5[m]; }
     }
+    const int lo = p->d0_bound[2 * dir], hi = p->d0_bound[2 * dir + 1];
+    /* non-reflecting flags (put_du_p/m_kernel, OCFD_Schemes.cpp:221-296): no P update of
+     * the first cell, no M update of the last, when that side has the closure */
+    const int skipP = lo && p->non_ref[2 * dir], skipM = hi && p->non_ref[2 * dir + 1];
     for (m = 0; m < 5; m++) {
-        for (f = 0; f <= n; f++) hP[f] = orc_cell_weno7_symbo_p(&fpv[m * np + f]);
+        for (f = 0; f <= n; f++)
+            if (!closure_p(&fpv[m * np + LAP], n, f, lo, hi, &hP[f])) hP[f] = orc_cell_weno7_symbo_p(&fpv[m * np + f]);
         for (c = 0; c < n; c++) {
             size_t g = at[c + LAP];
+            if (skipP && c == 0) continue;
             du[du_at[c] + m] = orc_cell_flux_diff(du[du_at[c] + m], ajac[g], hP[c + 1], hP[c], h);
         }
     }
     for (m = 0; m < 5; m++) {
-        for (f = 0; f <= n; f++) hM[f] = orc_cell_weno7_symbo_m(&fmv[m * np + f]);
+        for (f = 0; f <= n; f++)
+            if (!closure_m(&fmv[m * np + LAP], n, f, lo, hi, &hM[f])) hM[f] = orc_cell_weno7_symbo_m(&fmv[m * np + f]);
         for (c = 0; c < n; c++) {
             size_t g = at[c + LAP];
+            if (skipM && c == n - 1) continue;
             du[du_at[c] + m] = orc_cell_flux_diff(du[du_at[c] + m], ajac[g], hM[c + 1], hM[c], h);
         }
     }
@@ -241,7 +438,7 @@ static void rhs_direction(const orc_params *p, int dir,
                 du_at[c] = (size_t)(5 * c);
                 for (m = 0; m < 5; m++) dl[5 * c + m] = du[base + (size_t)m * N];
             }
-            rhs_line(p, n, at, du_at, h, d, u, v, w, cc, A1, A2, A3, ajac, dl, buf);
+            rhs_line(p, dir, n, at, du_at, h, d, u, v, w, cc, A1, A2, A3, ajac, dl, buf);
             for (c = 0; c < n; c++) {
                 size_t base = dir == 0 ? S(p, c, a1, a2, 0) : dir == 1 ? S(p, a1, c, a2, 0) : S(p, a1, a2, c, 0);
                 for (m = 0; m < 5; m++) du[base + (size_t)m * N] = dl[5 * c + m];

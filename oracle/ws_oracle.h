@@ -29,6 +29,12 @@ typedef struct orc_params {
     int nthreads;            /* OpenMP threads, 0 = runtime default */
     int viscous;             /* 0: inviscid RHS only; 6 or 8: add the viscous terms with CD6 / CD8 */
     double re, pr, ref_t;    /* Re, Pr, Ref_T of the viscous terms */
+    int periodic[3];         /* 1: ghost fill copies periodic images in that direction; 0: the
+                                caller's ghost values stay (Iperiodic) */
+    int d0_bound[6];         /* 1: one-sided closure of the inviscid scheme at that domain face
+                                (x-, x+, y-, y+, z-, z+; the reference's D0_bound) */
+    int non_ref[6];          /* 1: non-reflecting flag of the P (even index) / M (odd) pass of a
+                                direction (the reference's Non_ref) */
 } orc_params;
 
 /* scalar kernels (for closed-form tests) */
@@ -44,6 +50,7 @@ void orc_cons_to_pri(const orc_params *p, const double *f,
 void orc_pri_to_cons_box(const orc_params *p, double *f,
                          const double *d, const double *u, const double *v, const double *w, const double *T,
                          const int start[3], const int end[3]);
+/* Periodic face ghosts of the directions with p->periodic[d] = 1 (others untouched). */
 void orc_ghost_fill_periodic(const orc_params *p, double *fld);
 void orc_sound_speed(const orc_params *p, const double *T, double *cc);
 
@@ -101,6 +108,12 @@ void orc_amu(const orc_params *p, double re, double ref_t, const double *T, doub
 void orc_viscous_rhs(const orc_params *p, double re, double pr, double ref_t, int order,
                      const double *u, const double *v, const double *w, const double *T,
                      const double *const metrics[9], const double *ajac, double *du);
+
+/* Face value of the one-sided closures (D0_bound) on a line of n cells: s points at cell 0
+ * (LAP ghost cells before it and after cell n-1); returns 1 and the P / M face value of
+ * face f (between cells f-1 and f) when the closure covers that face, else 0. */
+int orc_closure_p(const double *s, long n, long f, int lo, int hi, double *out);
+int orc_closure_m(const double *s, long n, long f, int lo, int hi, double *out);
 
 /* Internals of one P reconstruction (for first-principles tests): smoothness
  * indicators after the downwind max, the unnormalised product-form weights
