@@ -103,7 +103,8 @@ typedef struct ws_metric_fields {
  * symbols (src/runtime/parameters_d.cpp:16-31, OCFD_init.cpp:29-31,379-396). */
 typedef struct ws_config {
     int32_t nx, ny, nz;          /* interior extents of the block (one GPU: the domain) */
-    int32_t periodic[3];         /* 1 = periodic in that direction (the only mode of this build) */
+    int32_t periodic[3];         /* 1: periodic (ghost fill copies periodic images); 0: the caller owns the
+                                    ghost layers of that direction (the reference's Iperiodic) */
     double gamma;                /* Gamma */
     double ama;                  /* Ama (reference Mach number) */
     double epsl_sw;              /* EPSL_SW, Steger-Warming smoothing */
@@ -116,6 +117,12 @@ typedef struct ws_config {
     int32_t viscous;             /* 0: inviscid RHS only; 6 or 8: ws_du_comput adds the viscous terms with
                                     CD6 / CD8 central differences (Scheme_vis_ID 203 / 204) */
     double re, pr, ref_t;        /* Re, Pr, Ref_T of the viscous terms (Sutherland viscosity) */
+    int32_t d0_bound[6];         /* 1: one-sided closure of the inviscid scheme at that domain face (x-, x+,
+                                    y-, y+, z-, z+; the reference's D0_bound, src/boundary/OCFD_bound_Scheme.cpp:
+                                    432-893); needs periodic[d] = 0 and at least 8 cells */
+    int32_t non_ref[6];          /* 1: non-reflecting flag of the P (even index) / M (odd) pass of a
+                                    direction (the reference's Non_ref: the P pass skips the first cell, the M
+                                    pass the last, where that side has the closure) */
 } ws_config;
 
 typedef struct ws_ctx ws_ctx;

@@ -64,8 +64,10 @@ def test_host_side_validation_without_gpu(ws):
     bad = ws.default_config(3, 32, 16)                  # extent below the halo depth
     assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -1
     assert b"LAP" in L.ws_last_error()
-    bad = ws.default_config(64, 32, 16)
-    bad.periodic[1] = 0                                 # non-periodic: not in this build
+    bad = ws.default_config(64, 32, 16, d0_bound=(0, 0, 1, 0, 0, 0))   # closure on a periodic direction
+    assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -1
+    assert b"non-periodic" in L.ws_last_error()
+    bad = ws.default_config(64, 32, 16, periodic=(1, 0, 1), viscous=6)  # viscous terms need periodic
     assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -2
     bad = ws.default_config(64, 32, 16, gamma=1.0)
     assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -1

@@ -54,7 +54,7 @@ class WsConfig(ctypes.Structure):
                 ("dt", ctypes.c_double), ("hx", ctypes.c_double), ("hy", ctypes.c_double),
                 ("hz", ctypes.c_double), ("metric", ctypes.c_double * 10), ("precision", _i32),
                 ("kernel", _i32), ("bc_planes", _i32), ("viscous", _i32), ("re", ctypes.c_double),
-                ("pr", ctypes.c_double), ("ref_t", ctypes.c_double)]
+                ("pr", ctypes.c_double), ("ref_t", ctypes.c_double), ("d0_bound", _i32 * 6), ("non_ref", _i32 * 6)]
 
 
 class WsState(ctypes.Structure):
@@ -190,9 +190,10 @@ def default_config(nx, ny, nz, **kw):
     c = WsConfig()
     lib().ws_config_default(ctypes.byref(c), nx, ny, nz)
     for k, v in kw.items():
-        if k == "metric":
+        if k in ("metric", "periodic", "d0_bound", "non_ref"):
+            arr = getattr(c, k)
             for i, x in enumerate(v):
-                c.metric[i] = x
+                arr[i] = x
         else:
             setattr(c, k, v)
     return c

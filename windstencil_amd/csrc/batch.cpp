@@ -98,6 +98,9 @@ int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *exten
 {
     if (!base || !extents || !states || !out || nblocks <= 0)
         return set_error(WS_ERR_INVALID, "bad batch request");
+    for (int i = 0; i < 3; ++i)
+        if (base->periodic[i] != 1 || base->d0_bound[2 * i] || base->d0_bound[2 * i + 1])
+            return set_error(WS_ERR_UNSUPPORTED, "batches hold periodic blocks only");
     *out = nullptr;
     ws_batch *b = new (std::nothrow) ws_batch;
     if (!b) return set_error(WS_ERR_NOMEM, "batch allocation failed");

@@ -47,6 +47,8 @@ KParams make_kparams(const ws_config &c)
         k.Ak[3 * dir] = a1 / sn; k.Ak[3 * dir + 1] = a2 / sn; k.Ak[3 * dir + 2] = a3 / sn;
     }
     k.ajac = c.metric[9];
+    for (int i = 0; i < 6; ++i) { k.bnd[i] = c.d0_bound[i] ? 1 : 0; k.nonref[i] = c.non_ref[i] ? 1 : 0; }
+    for (int d = 0; d < 3; ++d) k.bnd_any[d] = k.bnd[2 * d] | k.bnd[2 * d + 1];
     return k;
 }
 
@@ -117,9 +119,20 @@ int validate_config(const ws_config *cfg)
         return set_error(WS_ERR_INVALID, "every extent must be at least LAP = 4 (periodic halo)");
     if ((long)cfg->nx * cfg->ny * cfg->nz > (1L << 33))
         return set_error(WS_ERR_INVALID, "grid too large");
-    for (int i = 0; i < 3; ++i)
-        if (cfg->periodic[i] != 1)
-            return set_error(WS_ERR_UNSUPPORTED, "only periodic boundaries are implemented");
+    const int n[3] = {cfg->nx, cfg->ny, cfg->nz};
+    for (int i = 0; i < 3; ++i) {
+        if (cfg->periodic[i] != 0 && cfg->periodic[i] != 1)
+            return set_error(WS_ERR_INVALID, "periodic[d] must be 0 (caller's ghosts) or 1");
+        for (int side = 0; side < 2; ++side) {
+            const int b = cfg->d0_bound[2 * i + side], nr = cfg->non_ref[2 * i + side];
+            if ((b != 0 && b != 1) || (nr != 0 && nr != 1))
+                return set_error(WS_ERR_INVALID, "d0_bound and non_ref entries must be 0 or 1");
+            if (b && cfg->periodic[i])
+                return set_error(WS_ERR_INVALID, "a one-sided closure (d0_bound) needs a non-periodic direction");
+        }
+        if ((cfg->d0_bound[2 * i] || cfg->d0_bound[2 * i + 1]) && n[i] < 8)
+            return set_error(WS_ERR_INVALID, "a direction with one-sided closures needs at least 8 cells");
+    }
     if (!(cfg->gamma > 1.0) || !(cfg->ama > 0.0) || !(cfg->epsl_sw >= 0.0))
         return set_error(WS_ERR_INVALID, "gamma must exceed 1, Ma must be positive, epsl_sw non-negative");
     if (!(cfg->hx > 0.0) || !(cfg->hy > 0.0) || !(cfg->hz > 0.0))
@@ -137,6 +150,8 @@ int validate_config(const ws_config *cfg)
         return set_error(WS_ERR_INVALID, "viscous must be 0 (off), 6 or 8 (central-difference order)");
     if (cfg->viscous && !(cfg->re > 0.0 && cfg->pr > 0.0 && cfg->ref_t > 0.0))
         return set_error(WS_ERR_INVALID, "viscous terms need positive Re, Pr and Ref_T");
+    if (cfg->viscous && !(cfg->periodic[0] && cfg->periodic[1] && cfg->periodic[2]))
+        return set_error(WS_ERR_UNSUPPORTED, "the viscous terms are built for periodic directions only");
     return WS_OK;
 }
 

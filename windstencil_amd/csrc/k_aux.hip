@@ -16,13 +16,17 @@ struct FieldList {
     ws_field f[8];
 };
 
-// Periodic LAP-deep face halo.  Linear index over the six face slabs of one field:
+// Periodic LAP-deep face halo of the periodic directions.  Linear index over the face
+// slabs of one field:
 // x slabs (2*LAP*ny*nz), y slabs (2*LAP*nx*nz), z slabs (2*LAP*nx*ny).
 // Ghost points read only interior points, so the three directions need no ordering.
-__global__ void __launch_bounds__(NT) k_ghost_fill(FieldList fl, int nx, int ny, int nz, int nyg)
+__global__ void __launch_bounds__(NT) k_ghost_fill(FieldList fl, int nx, int ny, int nz, int nyg, int px, int py,
+                                                   int pz)
 {
     const ws_field f = fl.f[blockIdx.y];
-    const long nxs = 2L * LAP * ny * nz, nys = 2L * LAP * nx * nz, nzs = 2L * LAP * nx * ny;
+    // slabs of non-periodic directions are left to the caller
+    const long nxs = px ? 2L * LAP * ny * nz : 0, nys = py ? 2L * LAP * nx * nz : 0;
+    const long nzs = pz ? 2L * LAP * nx * ny : 0;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int dx, dy, dz, sx, sy, sz;
     if (i < nxs) {
@@ -225,12 +229,16 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
 int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s)
 {
     const KParams &k = c.kp;
-    const long per = 2L * LAP * ((long)k.ny * k.nz + (long)k.nx * k.nz + (long)k.nx * k.ny);
+    const int px = c.cfg.periodic[0], py = c.cfg.periodic[1], pz = c.cfg.periodic[2];
+    const long per = 2L * LAP * ((px ? (long)k.ny * k.nz : 0) + (py ? (long)k.nx * k.nz : 0) +
+                                 (pz ? (long)k.nx * k.ny : 0));
+    if (per == 0) return WS_OK;
     for (int base = 0; base < n; base += 8) {
         FieldList fl{};
         const int cnt = n - base < 8 ? n - base : 8;
         for (int i = 0; i < cnt; ++i) fl.f[i] = f[base + i];
-        hipLaunchKernelGGL(k_ghost_fill, dim3(blocks_for(per), cnt), dim3(NT), 0, s, fl, k.nx, k.ny, k.nz, k.nyg);
+        hipLaunchKernelGGL(k_ghost_fill, dim3(blocks_for(per), cnt), dim3(NT), 0, s, fl, k.nx, k.ny, k.nz, k.nyg,
+                           px, py, pz);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "ghost fill launch");

@@ -103,21 +103,32 @@ __global__ void __launch_bounds__(256) k_anchor_weno(KParams k, ws_metric_fields
     // face c-1/2 uses points cpos-4 .. cpos+3, i.e. line indices cpos .. cpos+7
     const size_t o = (size_t)x - LAP + (size_t)du.pitch * ((size_t)(y - LAP) + (size_t)k.ny * (size_t)(z - LAP));
     const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
+    // one-sided closures at closed domain faces: global face indices of this cell's faces
+    const int n_dom = DIR == 0 ? k.nx : DIR == 1 ? k.ny : k.nz;
+    const int g0 = box.s[DIR] - LAP;                 // global index of the box's first cell
+    const int fl = g0 + cpos, fr = fl + 1;
+    const bool lo = k.bnd[2 * DIR] != 0, hi = k.bnd[2 * DIR + 1] != 0;
+    const bool skipP = lo && k.nonref[2 * DIR] && fl == 0;
+    const bool skipM = hi && k.nonref[2 * DIR + 1] && fl == n_dom - 1;
 #pragma unroll 1
     for (int m = 0; m < 5; ++m) {
         const double *q = scratch + (size_t)m * plane + line;
-        const double hr = weno_ref<false>(q + cpos + 1);
-        const double hl = weno_ref<false>(q + cpos);
+        auto cell = [&](int cg) { return q[cg - g0 + LAP]; };
+        double hr, hl;
+        if (!(k.bnd_any[DIR] && closure_p(cell, n_dom, fr, lo, hi, hr))) hr = weno_ref<false>(q + cpos + 1);
+        if (!(k.bnd_any[DIR] && closure_p(cell, n_dom, fl, lo, hi, hl))) hl = weno_ref<false>(q + cpos);
         double *dp = du.ptr + o + (size_t)m * vstride;
-        *dp = *dp + -aj * (hr - hl) / h;
+        if (!skipP) *dp = *dp + -aj * (hr - hl) / h;
     }
 #pragma unroll 1
     for (int m = 0; m < 5; ++m) {
         const double *q = scratch + (size_t)(5 + m) * plane + line;
-        const double hr = weno_ref<true>(q + cpos + 1);
-        const double hl = weno_ref<true>(q + cpos);
+        auto cell = [&](int cg) { return q[cg - g0 + LAP]; };
+        double hr, hl;
+        if (!(k.bnd_any[DIR] && closure_m(cell, n_dom, fr, lo, hi, hr))) hr = weno_ref<true>(q + cpos + 1);
+        if (!(k.bnd_any[DIR] && closure_m(cell, n_dom, fl, lo, hi, hl))) hl = weno_ref<true>(q + cpos);
         double *dp = du.ptr + o + (size_t)m * vstride;
-        *dp = *dp + -aj * (hr - hl) / h;
+        if (!skipM) *dp = *dp + -aj * (hr - hl) / h;
     }
 }
 

@@ -156,7 +156,7 @@ __device__ __forceinline__ void tile_load(int nyg, const ws_field &d, const ws_f
 // split the loaded points into LDS, call next() (the persistent kernels issue the next
 // tile's loads there, into the registers just consumed), reconstruct the faces, then
 // write the flux difference of every cell.  Every thread of the block calls it.
-template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, typename Next>
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND, typename Next>
 __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_fields &mf, const ws_soa &du,
                                              const TileGeo &g, TileLoads<DIR, L, W, K> &ld, double *sp,
                                              Next next)
@@ -228,12 +228,35 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     const int f0 = seg * K;                               // first face of this thread
     const bool active = (t < TL::NT) && (line >= g.lo_across) && (line < g.lim_across) && (f0 <= g.lim_line);
     double Fv[5][K];
+    // one-sided closures (BND): global index of the tile's first cell along the line
+    const int n_dom = DIR == 0 ? k.nx : DIR == 1 ? k.ny : k.nz;
+    const int gl0 = (DIR == 0 ? g.x0 : DIR == 1 ? g.y0 : g.z0) - LAP;
+    const bool lo = BND && k.bnd[2 * DIR], hi = BND && k.bnd[2 * DIR + 1];
+    const bool nr_lo = lo && k.nonref[2 * DIR], nr_hi = hi && k.nonref[2 * DIR + 1];
+    double nfix[BND ? 5 : 1];       // P(n) - P(n-1) of the line's last cell (non-reflecting M)
     if (active) {
         const int base = TL::pidx(line, f0);
+        const bool near = BND && ((lo && gl0 + f0 <= 3) || (hi && gl0 + f0 + K - 1 >= n_dom - 3));
 #pragma unroll 1
         for (int m = 0; m < 5; ++m) {
             const double *qp = sp + m * TL::QS + base;
             const double *qm = sp + (5 + m) * TL::QS + base;
+            auto cellP = [&](int cg) { return sp[m * TL::QS + TL::pidx(line, cg - gl0 + LAP)]; };
+            auto cellM = [&](int cg) { return sp[(5 + m) * TL::QS + TL::pidx(line, cg - gl0 + LAP)]; };
+            // faces covered by a closure: P + M with either side replaced (FP64 closures)
+            auto closed = [&](int j, auto wenoP, auto wenoM) {
+                const int fg = gl0 + f0 + j;
+                double pv, mv;
+                const bool cp = closure_p(cellP, n_dom, fg, lo, hi, pv);
+                const bool cm = closure_m(cellM, n_dom, fg, lo, hi, mv);
+                if (cp || cm) Fv[m][j] = (cp ? pv : wenoP()) + (cm ? mv : wenoM());
+                if (BND && nr_hi && fg == n_dom) {
+                    double a = 0.0, b = 0.0;
+                    closure_p(cellP, n_dom, n_dom - 1, lo, hi, a);
+                    closure_p(cellP, n_dom, n_dom, lo, hi, b);
+                    nfix[m] = b - a;
+                }
+            };
             // a stored face f (f <= lim_line) reads only points f..f+7, all written in
             // phase 1; faces past lim_line read stale LDS and are not stored
             if (MIX) {
@@ -244,6 +267,10 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                 weno_k_mixed<true, K>([&](int j) { return qm[j]; }, bM, cM);
 #pragma unroll
                 for (int j = 0; j < K; ++j) Fv[m][j] = (bP[j] + bM[j]) + (double)(cP[j] + cM[j]);
+                if (near)
+#pragma unroll
+                    for (int j = 0; j < K; ++j)
+                        closed(j, [&] { return bP[j] + (double)cP[j]; }, [&] { return bM[j] + (double)cM[j]; });
             } else {
                 double nP[K], dP[K], nM[K], dM[K];
                 weno_k<false, K>([&](int j) { return qp[j]; }, nP, dP);
@@ -253,17 +280,26 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                     const double den = dP[j] * dM[j];
                     Fv[m][j] = fma(nP[j], dM[j], nM[j] * dP[j]) * recip_nr(den);
                 }
+                if (near)
+#pragma unroll
+                    for (int j = 0; j < K; ++j)
+                        closed(j, [&] { return nP[j] / dP[j]; }, [&] { return nM[j] / dM[j]; });
             }
         }
     }
     __syncthreads();
-    // face fluxes reuse the split buffer
+    // face fluxes reuse the split buffer (its first 5*W*F entries; the M split values, from
+    // 5*QS on, stay intact for the non-reflecting fix of phase 3)
+    double *nfix_lds = sp + 5 * W * TL::F;        // [W][5], free space below 5*QS
     if (active) {
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (f0 + j > g.lim_line || f0 + j >= TL::F) break;
 #pragma unroll
             for (int m = 0; m < 5; ++m) sp[m * W * TL::F + TL::fidx(line, f0 + j)] = Fv[m][j];
+            if (BND && nr_hi && gl0 + f0 + j == n_dom)
+#pragma unroll
+                for (int m = 0; m < 5; ++m) nfix_lds[line * 5 + m] = nfix[m];
         }
     }
     __syncthreads();
@@ -287,16 +323,28 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
         double *dp = du.ptr + (size_t)(cx - LAP) +
                      (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
         const int fl = TL::fidx(l, i), fr = TL::fidx(l, i + 1);
+        // non-reflecting flags: the first cell takes only the M pass, the last only the P pass
+        const bool first_only_m = BND && nr_lo && gl0 + i == 0;
+        const bool last_only_p = BND && nr_hi && gl0 + i == n_dom - 1;
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-            const double val = scale * (sp[m * W * TL::F + fr] - sp[m * W * TL::F + fl]);
+            double val = scale * (sp[m * W * TL::F + fr] - sp[m * W * TL::F + fl]);
+            if (first_only_m) {
+                auto cellM = [&](int cg) { return sp[(5 + m) * TL::QS + TL::pidx(l, cg - gl0 + LAP)]; };
+                double a = 0.0, b = 0.0;
+                closure_m(cellM, n_dom, 0, lo, hi, a);
+                closure_m(cellM, n_dom, 1, lo, hi, b);
+                val = scale * (b - a);
+            } else if (last_only_p) {
+                val = scale * nfix_lds[l * 5 + m];
+            }
             if (ACC) dp[m * vstride] = duo[r][m] + val;
             else dp[m * vstride] = val;
         }
     }
 }
 
-template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX>
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND>
 __global__ void __launch_bounds__(320)
 k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_metric_fields mf,
            ws_soa du, Box box)
@@ -306,7 +354,7 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
     const TileGeo g = tile_geo<DIR, L, W, K>(box, blockIdx.x, blockIdx.y, blockIdx.z);
     TileLoads<DIR, L, W, K> ld;
     tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
-    tile_compute<DIR, L, W, K, FIELDS, ACC, MIX>(k, mf, du, g, ld, sp, [] {});
+    tile_compute<DIR, L, W, K, FIELDS, ACC, MIX, BND>(k, mf, du, g, ld, sp, [] {});
 }
 
 // Persistent form for one box: the resident work-groups each walk a contiguous run of
@@ -338,7 +386,7 @@ k_rhs_persist(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_fiel
         auto next = [&] {
             if (PREFETCH && more) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, gn, ld);
         };
-        tile_compute<DIR, L, W, K, false, ACC, MIX>(k, nomf, du, g, ld, sp, next);
+        tile_compute<DIR, L, W, K, false, ACC, MIX, false>(k, nomf, du, g, ld, sp, next);
         __syncthreads();   // the next tile reuses the LDS buffer
         g = gn;
     }
@@ -397,7 +445,7 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
                 tile_load<DIR, L, W, K>(bn.n[1] + 2 * LAP, bn.d, bn.u, bn.v, bn.w, bn.cc, gn, ld);
             }
         };
-        tile_compute<DIR, L, W, K, false, ACC, MIX>(kb, nomf, b.du, g, ld, sp, next);
+        tile_compute<DIR, L, W, K, false, ACC, MIX, false>(kb, nomf, b.du, g, ld, sp, next);
         __syncthreads();   // the next tile reuses the LDS buffer
         tl = tn;
         g = gn;
@@ -425,7 +473,7 @@ k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles)
     TileLoads<DIR, L, W, K> ld;
     tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
     const ws_metric_fields nomf{};
-    tile_compute<DIR, L, W, K, false, ACC, MIX>(kb, nomf, b.du, g, ld, sp, [] {});
+    tile_compute<DIR, L, W, K, false, ACC, MIX, false>(kb, nomf, b.du, g, ld, sp, [] {});
 }
 
 // work-groups of a persistent kernel that fit on the device at once
@@ -467,9 +515,13 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     if (DIR == 0) grid = dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
     else if (DIR == 1) grid = dim3((nxa + W - 1) / W, (ny + L - 1) / L, nz);
     else grid = dim3((nxa + W - 1) / W, (nz + L - 1) / L, ny);
-    if (FIELDS || sched == WS_SCHED_GRID) {
-        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX>), grid, dim3(TL::NT), 0, s, k, d, u, v, w,
-                           cc, mf, du, box);
+    if (k.bnd_any[DIR]) {
+        // one-sided closures in this direction: the grid schedule with the closure code
+        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, true>), grid, dim3(TL::NT), 0, s, k, d, u, v,
+                           w, cc, mf, du, box);
+    } else if (FIELDS || sched == WS_SCHED_GRID) {
+        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, false>), grid, dim3(TL::NT), 0, s, k, d, u, v,
+                           w, cc, mf, du, box);
     } else if (sched == WS_SCHED_PREFETCH) {
         launch_persist<DIR, L, W, K, ACC, MIX, true>(k, d, u, v, w, cc, du, box, grid.x, grid.y, grid.z, s);
     } else {

@@ -352,6 +352,191 @@ __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (
     }
 }
 
+// ---- one-sided closures at non-periodic domain faces (the reference's D0_bound) ------
+// Reduced WENO5 reconstructions on 5-point windows s0..s4 without linear weights
+// (src/boundary/OCFD_bound_Scheme.cpp:540-719, src/kernels/OCFD_Schemes.cpp:1110-1201), in
+// the reference's operation order with contraction off.  The full P variant builds its
+// first two weights from S1 and S2 before they are assigned (both still 0); that is the
+// reference's behaviour and is kept.
+__device__ __forceinline__ double bs_sign1(double x) { return x >= 0 ? 1.0 : -1.0; }
+
+__device__ __forceinline__ double bs_minmod2(double a, double b)
+{
+#pragma clang fp contract(off)
+    return 0.5 * (bs_sign1(a) + bs_sign1(b)) * fmin(fabs(a), fabs(b));
+}
+
+__device__ __forceinline__ double w5_p(double s0, double s1, double s2, double s3, double s4)
+{
+#pragma clang fp contract(off)
+    const double ep = 1e-6;
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0, tmp;
+    tmp = s2 - 2.0 * s3 + s4; S0 += 13 * tmp * tmp;
+    tmp = 3.0 * s2 - 4.0 * s3 + s4; S0 += 3 * tmp * tmp;
+    const double a0 = ((12.0 * ep + S1) * (12.0 * ep + S1)) * ((12.0 * ep + S2) * (12.0 * ep + S2));
+    const double q03 = (2.0 * s2 + 5.0 * s3 - s4);
+    tmp = s1 - 2.0 * s2 + s3; S1 += 13 * tmp * tmp;
+    tmp = s1 - s3; S1 += 3 * tmp * tmp;
+    const double a1 = ((12.0 * ep + S0) * (12.0 * ep + S0)) * ((12.0 * ep + S2) * (12.0 * ep + S2));
+    const double q13 = (-s1 + 5.0 * s2 + 2.0 * s3);
+    tmp = s0 - 2.0 * s1 + s2; S2 += 13 * tmp * tmp;
+    tmp = s0 - 4.0 * s1 + 3.0 * s2; S2 += 3 * tmp * tmp;
+    const double a2 = ((12.0 * ep + S0) * (12.0 * ep + S0)) * ((12.0 * ep + S1) * (12.0 * ep + S1));
+    const double q23 = (2.0 * s0 - 7.0 * s1 + 11.0 * s2);
+    return (a0 * q03 + a1 * q13 + a2 * q23) / (6.0 * (a0 + a1 + a2));
+}
+
+__device__ __forceinline__ double w5_m(double s0, double s1, double s2, double s3, double s4)
+{
+#pragma clang fp contract(off)
+    const double ep = 1e-6;
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0, tmp;
+    tmp = s2 - 2.0 * s1 + s0; S0 += 13 * tmp * tmp;
+    tmp = 3.0 * s2 - 4.0 * s1 + s0; S0 += 3 * tmp * tmp;
+    const double a0 = 1.0 / ((12.0 * ep + S0) * (12.0 * ep + S0));
+    const double q03 = (2.0 * s2 + 5.0 * s1 - s0);
+    tmp = s3 - 2.0 * s2 + s1; S1 += 13 * tmp * tmp;
+    tmp = s3 - s1; S1 += 3 * tmp * tmp;
+    const double a1 = 1.0 / ((12.0 * ep + S1) * (12.0 * ep + S1));
+    const double q13 = (-s3 + 5.0 * s2 + 2.0 * s1);
+    tmp = s4 - 2.0 * s3 + s2; S2 += 13 * tmp * tmp;
+    tmp = s4 - 4.0 * s3 + 3.0 * s2; S2 += 3 * tmp * tmp;
+    const double a2 = 1.0 / ((12.0 * ep + S2) * (12.0 * ep + S2));
+    const double q23 = (2.0 * s4 - 7.0 * s3 + 11.0 * s2);
+    return (a0 * q03 + a1 * q13 + a2 * q23) / (6.0 * (a0 + a1 + a2));
+}
+
+// single candidate k (weight a_k = 1/(12 ep + S_k)^2 cancels); q is the candidate, S its smoothness
+__device__ __forceinline__ double w5_one(double S, double q)
+{
+#pragma clang fp contract(off)
+    const double ep = 1e-6;
+    const double a = 1.0 / ((12.0 * ep + S) * (12.0 * ep + S));
+    return a * q / (6.0 * a);
+}
+
+__device__ __forceinline__ double w5_two(double Sa, double qa, double Sb, double qb)
+{
+#pragma clang fp contract(off)
+    const double ep = 1e-6;
+    const double a = 1.0 / ((12.0 * ep + Sa) * (12.0 * ep + Sa));
+    const double b = 1.0 / ((12.0 * ep + Sb) * (12.0 * ep + Sb));
+    return (a * qa + b * qb) / (6.0 * (a + b));
+}
+
+// smoothness of the three WENO5 candidates, in the reference's expression order
+__device__ __forceinline__ double w5_S0p(double s2, double s3, double s4)
+{
+#pragma clang fp contract(off)
+    double S = 0.0, tmp;
+    tmp = s2 - 2.0 * s3 + s4; S += 13 * tmp * tmp;
+    tmp = 3.0 * s2 - 4.0 * s3 + s4; S += 3 * tmp * tmp;
+    return S;
+}
+__device__ __forceinline__ double w5_S1p(double s1, double s2, double s3)
+{
+#pragma clang fp contract(off)
+    double S = 0.0, tmp;
+    tmp = s1 - 2.0 * s2 + s3; S += 13 * tmp * tmp;
+    tmp = s1 - s3; S += 3 * tmp * tmp;
+    return S;
+}
+__device__ __forceinline__ double w5_S2p(double s0, double s1, double s2)
+{
+#pragma clang fp contract(off)
+    double S = 0.0, tmp;
+    tmp = s0 - 2.0 * s1 + s2; S += 13 * tmp * tmp;
+    tmp = s0 - 4.0 * s1 + 3.0 * s2; S += 3 * tmp * tmp;
+    return S;
+}
+
+// P face f (between cells f-1 and f) of a line of n cells; cell(c) gives the split value
+// of cell c.  Returns true and the face value when a closure covers the face
+// (OCFD_bound_scheme_kernel_p, OCFD_bound_Scheme.cpp:722-807; 5-point windows on cells
+// f-3 .. f+1).
+template <typename Cell>
+__device__ __forceinline__ bool closure_p(Cell cell, int n, int f, bool lo, bool hi, double &out)
+{
+#pragma clang fp contract(off)
+    if (lo && f <= 3) {
+        if (f == 0) {
+            out = cell(0);
+        } else if (f == 1) {            // lift: candidate 0 on cells f-1 .. f+1
+            const double s2 = cell(f - 1), s3 = cell(f), s4 = cell(f + 1);
+            out = w5_one(w5_S0p(s2, s3, s4), (2.0 * s2 + 5.0 * s3 - s4));
+        } else if (f == 2) {            // lift_plus: candidates 0 and 1
+            const double s1 = cell(f - 2), s2 = cell(f - 1), s3 = cell(f), s4 = cell(f + 1);
+            out = w5_two(w5_S0p(s2, s3, s4), (2.0 * s2 + 5.0 * s3 - s4), w5_S1p(s1, s2, s3),
+                         (-s1 + 5.0 * s2 + 2.0 * s3));
+        } else {
+            out = w5_p(cell(f - 3), cell(f - 2), cell(f - 1), cell(f), cell(f + 1));
+        }
+        return true;
+    }
+    if (hi && f >= n - 2) {
+        if (f == n) {
+            const double a = cell(n - 1), b = cell(n - 2);
+            out = a + 0.5 * bs_minmod2(a - b, a - b);
+        } else if (f == n - 1) {        // right_plus: candidates 1 and 2
+            const double s0 = cell(f - 3), s1 = cell(f - 2), s2 = cell(f - 1), s3 = cell(f);
+            out = w5_two(w5_S1p(s1, s2, s3), (-s1 + 5.0 * s2 + 2.0 * s3), w5_S2p(s0, s1, s2),
+                         (2.0 * s0 - 7.0 * s1 + 11.0 * s2));
+        } else {
+            out = w5_p(cell(f - 3), cell(f - 2), cell(f - 1), cell(f), cell(f + 1));
+        }
+        return true;
+    }
+    return false;
+}
+
+// M face f (OCFD_bound_scheme_kernel_m, OCFD_bound_Scheme.cpp:808-893; windows on cells
+// f-2 .. f+2, mirrored candidates)
+template <typename Cell>
+__device__ __forceinline__ bool closure_m(Cell cell, int n, int f, bool lo, bool hi, double &out)
+{
+#pragma clang fp contract(off)
+    if (lo && f <= 3) {
+        if (f == 0) {                   // lift: candidate 2 on cells f .. f+2
+            const double s2 = cell(f), s3 = cell(f + 1), s4 = cell(f + 2);
+            double S = 0.0, tmp;
+            tmp = s4 - 2.0 * s3 + s2; S += 13 * tmp * tmp;
+            tmp = s4 - 4.0 * s3 + 3.0 * s2; S += 3 * tmp * tmp;
+            out = w5_one(S, (2.0 * s4 - 7.0 * s3 + 11.0 * s2));
+        } else if (f == 1) {            // lift_plus: candidates 1 and 2
+            const double s1 = cell(f - 1), s2 = cell(f), s3 = cell(f + 1), s4 = cell(f + 2);
+            double S1 = 0.0, S2 = 0.0, tmp;
+            tmp = s3 - 2.0 * s2 + s1; S1 += 13 * tmp * tmp;
+            tmp = s3 - s1; S1 += 3 * tmp * tmp;
+            tmp = s4 - 2.0 * s3 + s2; S2 += 13 * tmp * tmp;
+            tmp = s4 - 4.0 * s3 + 3.0 * s2; S2 += 3 * tmp * tmp;
+            out = w5_two(S1, (-s3 + 5.0 * s2 + 2.0 * s1), S2, (2.0 * s4 - 7.0 * s3 + 11.0 * s2));
+        } else {
+            out = w5_m(cell(f - 2), cell(f - 1), cell(f), cell(f + 1), cell(f + 2));
+        }
+        return true;
+    }
+    if (hi && f >= n - 3) {
+        if (f == n) {
+            out = cell(n - 1);
+        } else if (f == n - 1) {
+            const double a = cell(n - 1), b = cell(n - 2);
+            out = a - 0.5 * bs_minmod2(a - b, a - b);
+        } else if (f == n - 2) {        // right_plus: candidates 0 and 1
+            const double s0 = cell(f - 2), s1 = cell(f - 1), s2 = cell(f), s3 = cell(f + 1);
+            double S0 = 0.0, S1 = 0.0, tmp;
+            tmp = s2 - 2.0 * s1 + s0; S0 += 13 * tmp * tmp;
+            tmp = 3.0 * s2 - 4.0 * s1 + s0; S0 += 3 * tmp * tmp;
+            tmp = s3 - 2.0 * s2 + s1; S1 += 13 * tmp * tmp;
+            tmp = s3 - s1; S1 += 3 * tmp * tmp;
+            out = w5_two(S0, (2.0 * s2 + 5.0 * s1 - s0), S1, (-s3 + 5.0 * s2 + 2.0 * s1));
+        } else {
+            out = w5_m(cell(f - 2), cell(f - 1), cell(f), cell(f + 1), cell(f + 2));
+        }
+        return true;
+    }
+    return false;
+}
+
 // Steger-Warming split in the form the tiled kernels use: A1..A3 the metric terms,
 // sn = |A|, k1..k3 = A/|A| supplied by the caller (constants when metrics are uniform).
 __device__ __forceinline__ void split_fast(const KParams &k, double d, double u, double v, double w,
