@@ -118,4 +118,4 @@ def test_batch_with_viscous_terms_equals_single_blocks(ws):
     torch.cuda.synchronize()
     for e, pr_, st in zip(extents, prims, b.states):
         ref = gpu_du_comput(make_ctx(*e, **_vis_cfg(8)), upload_state(pr_))
-        assert np.array_equal(_du(st), ref), e
+        assert rel_l2(_du(st), ref) <= 1e-13, e
