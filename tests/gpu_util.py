@@ -29,7 +29,8 @@ def upload_state(prims, pad=0):
 
 def gpu_du_comput(ctx, st, metrics=None, job=None, accumulate=False, stream=None):
     """Ghost fill d,u,v,w,T; sound speed over the ghosted box; RHS over `job` (default:
-    the interior) into st.du.  Returns du (numpy, interior)."""
+    the interior) into st.du, plus the viscous terms when the context has them (as
+    ws_du_comput).  Returns du (numpy, interior)."""
     import torch
     L = W.lib()
     s = W.stream_handle(stream)
@@ -42,6 +43,9 @@ def gpu_du_comput(ctx, st, metrics=None, job=None, accumulate=False, stream=None
     mp = ctypes.byref(metrics) if metrics is not None else None
     W.check(L.ws_inviscid_rhs(ctx.handle, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.cc, mp, st.ws.du, j,
                               1 if accumulate else 0, s), "inviscid rhs")
+    if ctx.cfg.viscous:
+        # du_comput adds the viscous terms after the inviscid RHS
+        W.check(L.ws_viscous_rhs(ctx.handle, st.ws.u, st.ws.v, st.ws.w, st.ws.T, mp, st.ws.du, s), "viscous rhs")
     torch.cuda.synchronize()
     return st.interior("du").cpu().numpy()
 

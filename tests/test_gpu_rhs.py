@@ -87,7 +87,8 @@ def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
 def test_schedules_agree(ws, tile, precision):
     """The grid, persistent and prefetching schedules run the same per-tile code (the
     compiler may contract multiply-adds differently per kernel): their RHS agree to
-    1e-13 relative L2, on the full box and on a sub-box job."""
+    1e-13 relative L2 in FP64 (1e-11 mixed), on the full box and on a sub-box job."""
+    tol = 1e-13 if precision == W.WS_PREC_FP64 else 1e-11
     for shape in [(31, 17, 19), (67, 45, 37), (130, 40, 33)]:
         nx, ny, nz = shape
         prims = random_state(nx, ny, nz, 41, "shock")
@@ -102,14 +103,14 @@ def test_schedules_agree(ws, tile, precision):
             subs.append(gpu_du_comput(ctx, st, job=W.job((4 + 3, 4 + 2, 4 + 1), (4 + nx - 2, 4 + ny - 3, 4 + nz - 1))))
         log(f"schedules {shape} tile {tile} precision {precision}")
         for o in outs[1:]:
-            assert rel_l2(o, outs[0]) <= 1e-13
+            assert rel_l2(o, outs[0]) <= tol
         inside = (slice(None), slice(1, nz - 1), slice(2, ny - 3), slice(3, nx - 2))
         mask = np.ones(subs[0].shape, bool)
         mask[inside] = False
         for o in subs:
             assert np.all(o[mask] == 3.0)
         for o in subs[1:]:
-            assert rel_l2(o[inside], subs[0][inside]) <= 1e-13
+            assert rel_l2(o[inside], subs[0][inside]) <= tol
 
 
 def test_tiled_is_independent_of_pitch(ws):
