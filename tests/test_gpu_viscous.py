@@ -1,8 +1,9 @@
 """GPU parity of the viscous terms (the reference's du_vis_Jacobian3d path) against the
-oracle.  The viscous kernels follow the reference's expression order with contraction
-off, so they must match the oracle bit for bit on their own; added to the tiled
-inviscid RHS the total stays within 1e-12 relative L2, and with the anchor inviscid
-kernels the whole RHS, and RK3 steps, are bit-identical."""
+oracle.  With the anchor kernel family the viscous kernels follow the reference's
+expression order with contraction off and its divisions, so they match the oracle bit
+for bit, alone, in the whole RHS and in RK3 steps.  The tiled family's viscous kernels
+scale each stencil sum by 1/(60 h) instead of dividing twice: within 1e-13 relative L2
+alone, and the total RHS within 1e-12."""
 import ctypes
 
 import numpy as np
@@ -29,9 +30,10 @@ def _oracle_full(orc, prims, order, metrics=None, ajac=None):
     return du
 
 
+@pytest.mark.parametrize("kernel", [W.WS_KERNEL_ANCHOR, W.WS_KERNEL_TILED])
 @pytest.mark.parametrize("order", [6, 8])
 @pytest.mark.parametrize("kind", ["blobs", "vortex", "rough"])
-def test_viscous_terms_bitwise(orc, ws, order, kind):
+def test_viscous_terms(orc, ws, order, kind, kernel):
     import torch
     nx, ny, nz = 33, 21, 17
     prims = random_state(nx, ny, nz, 3, kind)
@@ -40,7 +42,7 @@ def test_viscous_terms_bitwise(orc, ws, order, kind):
     for a in g:
         orc.ghost_fill_periodic(p, a)
     ref = orc.viscous_rhs(p, g[1], g[2], g[3], g[4], RE, PR, TREF, order)
-    ctx = make_ctx(nx, ny, nz, **_vis_cfg(order))
+    ctx = make_ctx(nx, ny, nz, kernel=kernel, **_vis_cfg(order))
     st = upload_state(prims)
     L = W.lib()
     s = W.stream_handle()
@@ -50,8 +52,12 @@ def test_viscous_terms_bitwise(orc, ws, order, kind):
     W.check(L.ws_viscous_rhs(ctx.handle, st.ws.u, st.ws.v, st.ws.w, st.ws.T, None, st.ws.du, s))
     torch.cuda.synchronize()
     du = st.interior("du").cpu().numpy()
-    log(f"viscous {kind} order {order}: max |diff| {np.abs(du - ref).max():.3e}")
-    assert np.array_equal(du, ref)
+    err = rel_l2(du, ref)
+    log(f"viscous {kind} order {order} kernel {kernel}: rel L2 {err:.3e}")
+    if kernel == W.WS_KERNEL_ANCHOR:
+        assert np.array_equal(du, ref)
+    else:
+        assert err <= 1e-13
 
 
 @pytest.mark.parametrize("order", [6, 8])
