@@ -264,18 +264,27 @@ int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
     if (!b) return set_error(WS_ERR_INVALID, "null batch");
     for (ws_state &st : b->st) std::swap(st.f, st.fn);
     const ws_soa nolap{};
-    for (int krk = 1; krk <= 3; ++krk) {
-        int r = ws_batch_du_comput(b, s);
-        if (r) return r;
-        for (size_t i = 0; i < b->ctx.size(); ++i) {
+    int r = WS_OK;
+    // the blocks are periodic (ws_batch_create); stages 1 and 2 leave the next stage's
+    // inputs prepared (fused stage kernel writing the sound speed, then one ghost fill)
+    for (int krk = 1; krk <= 3 && !r; ++krk) {
+        for (size_t i = 0; i < b->ctx.size() && !r && krk == 1; ++i) r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
+        if (!r) r = batch_rhs(b, s);
+        for (size_t i = 0; i < b->ctx.size() && !r && b->base.viscous; ++i) {
+            const ws_state &st = b->st[i];
+            r = launch_viscous(*b->ctx[i], st.u, st.v, st.w, st.T, nullptr, st.du, s);
+        }
+        for (size_t i = 0; i < b->ctx.size() && !r; ++i) {
             ws_ctx *c = b->ctx[i];
-            ws_state &st = b->st[i];
-            if ((r = launch_time_advance(*c, krk, st.f, st.fn, st.du, nolap, c->cfg.dt, dt_dev, s))) return r;
-            if ((r = launch_cons_to_pri(*c, st.f, st.d, st.u, st.v, st.w, st.T, st.P, s))) return r;
-            if (c->cfg.bc_planes && (r = ws_bc_planes(c, &st, s))) return r;
+            const ws_state &st = b->st[i];
+            r = launch_stage(*c, krk, st, nolap, c->cfg.dt, dt_dev, c->cfg.bc_planes != 0, krk < 3, s);
+            if (!r && krk < 3) {
+                const ws_field g[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
+                r = launch_ghost_fill(*c, g, 6, s);
+            }
         }
     }
-    return WS_OK;
+    return r;
 }
 
 int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t s)

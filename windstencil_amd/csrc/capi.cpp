@@ -220,6 +220,36 @@ int prepare_rhs_inputs(ws_ctx *c, const ws_state *st, hipStream_t s)
     return r;
 }
 
+int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s)
+{
+    int r = rhs_for_ctx(c, st, s);
+    if (!r && c->cfg.viscous) r = launch_viscous(*c, st->u, st->v, st->w, st->T, nullptr, st->du, s);
+    return r;
+}
+
+// The stage kernel of stages 1 and 2 also writes the interior sound speed, and one ghost
+// fill of the primitives and cc then prepares the next stage's RHS: the same values as the
+// separate ghost fill and ghosted-box sound speed on the faces the RHS reads (ghost cells
+// copy interior cells, whose sound speed is computed by the same expression).  With a
+// non-periodic direction the caller's ghost layers of T feed the sound speed there, so
+// every stage then prepares its inputs the separate way.
+int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
+{
+    std::swap(st->f, st->fn);
+    const ws_soa nolap{};
+    const bool periodic = c->cfg.periodic[0] && c->cfg.periodic[1] && c->cfg.periodic[2];
+    const ws_field g[6] = {st->d, st->u, st->v, st->w, st->T, st->cc};
+    int r = WS_OK;
+    for (int krk = 1; krk <= 3 && !r; ++krk) {
+        if (krk == 1 || !periodic) r = prepare_rhs_inputs(c, st, s);
+        if (!r) r = du_from_prepared(c, st, s);
+        const bool sound = periodic && krk < 3;
+        if (!r) r = launch_stage(*c, krk, *st, nolap, c->cfg.dt, dt_dev, c->cfg.bc_planes != 0, sound, s);
+        if (!r && sound) r = launch_ghost_fill(*c, g, 6, s);
+    }
+    return r;
+}
+
 }  // namespace ws
 
 extern "C" {
@@ -515,17 +545,9 @@ int ws_cfl_dt(ws_ctx *c, const ws_state *st, double cfl, double *dt_dev, hipStre
 int ws_rk3_step(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
 {
     if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
-    // pf <-> pfn (OCFD_NS_Solver.cpp:66-74): fn now holds the state at t^n
-    std::swap(st->f, st->fn);
-    const ws_soa nolap{};
-    for (int krk = 1; krk <= 3; ++krk) {
-        int r;
-        if ((r = ws_du_comput(c, st, nullptr, s))) return r;
-        if ((r = launch_time_advance(*c, krk, st->f, st->fn, st->du, nolap, c->cfg.dt, dt_dev, s))) return r;
-        if ((r = launch_cons_to_pri(*c, st->f, st->d, st->u, st->v, st->w, st->T, st->P, s))) return r;
-        if (c->cfg.bc_planes && (r = ws_bc_planes(c, st, s))) return r;
-    }
-    return WS_OK;
+    // pf <-> pfn (OCFD_NS_Solver.cpp:66-74): fn now holds the state at t^n; then per stage
+    // du_comput, OCFD_time_advance, get_duvwT and OCFD_bc, fused
+    return rk3_step_fused(c, st, dt_dev, s);
 }
 
 int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t s)

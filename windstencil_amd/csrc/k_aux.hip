@@ -155,6 +155,67 @@ __global__ void __launch_bounds__(NT) k_time_advance(KParams k, ws_soa f, ws_soa
     }
 }
 
+// One RK3 stage of a solver step, fused: the stage update of f (as k_time_advance), the
+// primitives of the new f (as k_cons_to_pri), with PLANES the re-derivation of f on the six
+// boundary planes from those primitives (as k_pri_to_cons over the planes), and with
+// SOUND the sound speed of the interior (as k_sound_speed; a ghost fill then copies it).
+// Same expressions with contraction off as the separate kernels, so the state is
+// bit-identical; f is read once and the primitives are not read back.
+template <int KRK, bool PLANES, bool SOUND>
+__global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl, double dt_val,
+                                              const double *dt_dev, ws_field d, ws_field u, ws_field v, ws_field w,
+                                              ws_field T, ws_field P, ws_field cc)
+{
+#pragma clang fp contract(off)
+    const long total = (long)k.nx * k.ny * k.nz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double dt = dt_dev ? *dt_dev : dt_val;
+    const int x = (int)(i % k.nx);
+    const int y = (int)((i / k.nx) % k.ny);
+    const int z = (int)(i / ((long)k.nx * k.ny));
+    double r[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double fnv = fn.ptr[sidx(fn.pitch, k.ny, k.nz, x, y, z, m)];
+        const double duv = du.ptr[sidx(du.pitch, k.ny, k.nz, x, y, z, m)];
+        if (KRK == 1) {
+            r[m] = fnv + dt * duv;
+        } else if (KRK == 2) {
+            const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
+            r[m] = t1 * fnv + t2 * (f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] + dt * duv);
+        } else {
+            const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
+            r[m] = t1 * fnv + t2 * (f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] + dt * duv);
+        }
+        if (KRK == 3 && fl.ptr)
+            fl.ptr[(size_t)(x + LAP) + (size_t)fl.pitch * ((size_t)(y + LAP) + (size_t)(k.ny + 2 * LAP) *
+                   ((size_t)(z + LAP) + (size_t)(k.nz + 2 * LAP) * m))] = r[m];
+    }
+    const double d1 = r[0];
+    const double u1 = r[1] / d1, v1 = r[2] / d1, w1 = r[3] / d1;
+    double T1 = r[4];
+    T1 = T1 - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1);
+    const double TT = T1 / (d1 * k.cv);
+    const int gx = x + LAP, gy = y + LAP, gz = z + LAP;
+    d.ptr[fidx(d.pitch, k.nyg, gx, gy, gz)] = d1;
+    u.ptr[fidx(u.pitch, k.nyg, gx, gy, gz)] = u1;
+    v.ptr[fidx(v.pitch, k.nyg, gx, gy, gz)] = v1;
+    w.ptr[fidx(w.pitch, k.nyg, gx, gy, gz)] = w1;
+    T.ptr[fidx(T.pitch, k.nyg, gx, gy, gz)] = TT;
+    if (P.ptr) P.ptr[fidx(P.pitch, k.nyg, gx, gy, gz)] = T1 / d1 * (k.gamma - 1.0);
+    if (SOUND) cc.ptr[fidx(cc.pitch, k.nyg, gx, gy, gz)] = sqrt(TT) / k.ama;
+    if (PLANES && (x == 0 || x == k.nx - 1 || y == 0 || y == k.ny - 1 || z == 0 || z == k.nz - 1)) {
+        r[0] = d1;
+        r[1] = u1 * d1;
+        r[2] = v1 * d1;
+        r[3] = w1 * d1;
+        r[4] = d1 * ((u1 * u1 + v1 * v1 + w1 * w1) * 0.5 + k.cv * TT);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] = r[m];
+}
+
 __global__ void __launch_bounds__(NT) k_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy,
                                              uint32_t sz)
 {
@@ -285,6 +346,35 @@ int launch_time_advance(const Ctx &c, int krk, ws_soa f, ws_soa fn, ws_soa du, w
     else hipLaunchKernelGGL(k_time_advance<3>, g, dim3(NT), 0, s, k, f, fn, du, fl, dt, dt_dev);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "time advance launch");
+}
+
+int launch_stage(const Ctx &c, int krk, const ws_state &st, ws_soa fl, double dt, const double *dt_dev, bool planes,
+                 bool sound, hipStream_t s)
+{
+    const KParams &k = c.kp;
+    const dim3 g(blocks_for((long)k.nx * k.ny * k.nz));
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, g, dim3(NT), 0, s, k, st.f, st.fn, st.du, fl, dt, dt_dev, st.d, st.u, st.v, st.w,
+                           st.T, st.P, st.cc);
+    };
+    const int sel = (krk - 1) * 4 + (planes ? 2 : 0) + (sound ? 1 : 0);
+    switch (sel) {
+    case 0: go(k_stage<1, false, false>); break;
+    case 1: go(k_stage<1, false, true>); break;
+    case 2: go(k_stage<1, true, false>); break;
+    case 3: go(k_stage<1, true, true>); break;
+    case 4: go(k_stage<2, false, false>); break;
+    case 5: go(k_stage<2, false, true>); break;
+    case 6: go(k_stage<2, true, false>); break;
+    case 7: go(k_stage<2, true, true>); break;
+    case 8: go(k_stage<3, false, false>); break;
+    case 9: go(k_stage<3, false, true>); break;
+    case 10: go(k_stage<3, true, false>); break;
+    case 11: go(k_stage<3, true, true>); break;
+    default: return set_error(WS_ERR_INVALID, "krk must be 1, 2 or 3");
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "stage launch");
 }
 
 int launch_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy, uint32_t sz, hipStream_t s)

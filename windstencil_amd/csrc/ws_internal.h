@@ -103,6 +103,12 @@ int check_state(const ws_ctx *c, const ws_state *st);
 int prepare_rhs_inputs(ws_ctx *c, const ws_state *st, hipStream_t s);
 // the RHS over the interior with the context's kernel and precision (overwrites du)
 int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s);
+// du_comput on inputs already prepared (ghosted primitives and sound speed): the inviscid
+// RHS, plus the viscous terms when the configuration has them
+int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s);
+// one SSP-RK3 step of st with fused stage kernels (the same arithmetic as du_comput,
+// time_advance, cons_to_pri and bc_planes in turn)
+int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s);
 
 // viscous terms added to du[1..4] (k_viscous.hip); allocates the context's scratch once
 int launch_viscous(Ctx &c, ws_field u, ws_field v, ws_field w, ws_field T, const ws_metric_fields *m, ws_soa du,
@@ -116,6 +122,10 @@ int launch_pri_to_cons(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field 
                        ws_field T, ws_job job, hipStream_t s);
 int launch_time_advance(const Ctx &c, int krk, ws_soa f, ws_soa fn, ws_soa du, ws_soa f_lap,
                         double dt, const double *dt_dev, hipStream_t s);
+// one fused RK3 stage on st: f from (f, fn, du), the primitives (and P) from the new f, with
+// planes the boundary-plane re-derivation of f, with sound the interior sound speed in cc
+int launch_stage(const Ctx &c, int krk, const ws_state &st, ws_soa f_lap, double dt, const double *dt_dev,
+                 bool planes, bool sound, hipStream_t s);
 int launch_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy, uint32_t sz,
                 hipStream_t s);
 int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream_t s);
