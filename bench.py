@@ -278,16 +278,13 @@ def main():
     st.set_prims(*prims)
     stream = torch.cuda.current_stream()
     sh = W.stream_handle(stream)
-    fl = (W.WsField * 5)(st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T)
-    gjob = W.job((0, 0, 0), (nx + 8, ny + 8, nz + 8))
     ijob = W.interior_job(nx, ny, nz)
 
     def rhs(ev=None):
-        # one full RHS: ghost fill, sound speed, three directional kernels
+        # one full RHS: ghost fill and sound speed (one pass), three directional kernels
         if ev is not None:
             ev[0].record(stream)
-        W.check(L.ws_ghost_fill_periodic(ctx.handle, fl, 5, sh))
-        W.check(L.ws_sound_speed(ctx.handle, st.ws.T, st.ws.cc, gjob, sh))
+        W.check(L.ws_prepare_rhs_inputs(ctx.handle, ctypes.byref(st.ws), sh))
         if ev is not None:
             ev[1].record(stream)
         W.check(L.ws_inviscid_rhs(ctx.handle, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.cc, None, st.ws.du,
@@ -374,8 +371,7 @@ def main():
         W.check(L.ws_ctx_set_schedule(vctx.handle, int(shape.split("/")[1])))
 
         def ns_rhs(ev=None):
-            W.check(L.ws_ghost_fill_periodic(vctx.handle, fl, 5, sh))
-            W.check(L.ws_sound_speed(vctx.handle, st.ws.T, st.ws.cc, gjob, sh))
+            W.check(L.ws_prepare_rhs_inputs(vctx.handle, ctypes.byref(st.ws), sh))
             W.check(L.ws_inviscid_rhs(vctx.handle, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.cc, None, st.ws.du,
                                       ijob, 0, sh))
             if ev is not None:

@@ -182,9 +182,17 @@ int ws_sound_speed(ws_ctx *ctx, ws_field T, ws_field cc, ws_job job, hipStream_t
  * (src/mpi/OCFD_mpi_dev.cpp:123-128,187-290; src/mpi/OCFD_mpi.c:171-197). */
 int ws_ghost_fill_periodic(ws_ctx *ctx, const ws_field *fields, int32_t nfields, hipStream_t stream);
 
-/* Full RHS of one stage from the state's primitives: periodic ghost fill of
- * d,u,v,w,T, sound speed over the ghosted box, inviscid RHS into du (overwritten).
- * Replaces du_comput's inviscid part (src/solver/OCFD_Stream.cpp:27-52,185-227). */
+/* The RHS inputs of a stage: the periodic face halo of d,u,v,w,T (as
+ * ws_ghost_fill_periodic), then cc = sqrt(T)/Ma over the whole ghosted box (as
+ * ws_sound_speed, as one flat pass over the field when T and cc share a layout),
+ * bit-identical to those two calls in turn.  Replaces the exchanges
+ * and du_invis_Jacobian3d_init at the head of du_invis_Jacobian3d
+ * (src/solver/OCFD_Stream.cpp:187-194, src/solver/OCFD_NS_Jacobian3d.cpp:24-33). */
+int ws_prepare_rhs_inputs(ws_ctx *ctx, const ws_state *st, hipStream_t stream);
+
+/* Full RHS of one stage from the state's primitives: ws_prepare_rhs_inputs, then the
+ * inviscid RHS into du (overwritten), then the viscous terms when the configuration has
+ * them.  Replaces du_comput (src/solver/OCFD_Stream.cpp:27-52,185-227). */
 int ws_du_comput(ws_ctx *ctx, const ws_state *st, const ws_metric_fields *metrics, hipStream_t stream);
 
 /* ---- state helpers ------------------------------------------------------- */

@@ -61,6 +61,46 @@ def test_ghost_fill_sound_speed_and_conversions_bitwise(orc, ws):
     assert np.array_equal(st2.t["cc"].cpu().numpy()[:, :, :nx + 8], cc_ref)
 
 
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 1), (1, 0, 0)])
+def test_prepare_rhs_inputs_equals_ghost_fill_then_sound_speed(ws, periodic):
+    """ws_prepare_rhs_inputs equals ws_ghost_fill_periodic of d,u,v,w,T followed by
+    ws_sound_speed over the ghosted box, bit for bit on every ghosted array (edge and
+    corner ghosts and non-periodic ghost layers keep the caller's values in both)."""
+    import torch
+    nx, ny, nz = 37, 22, 15
+    L = W.lib()
+    s = W.stream_handle()
+    cfg = dict(periodic=list(periodic))
+    if not all(periodic):
+        cfg.update(d0_bound=[1 - p for p in periodic for _ in (0, 1)])
+    ctx = make_ctx(nx, ny, nz, **cfg)
+    prims = random_state(nx, ny, nz, 6, "blobs")
+    runs = []
+    for fused in (False, True):
+        st = upload_state(prims)
+        gen = torch.Generator(device="cpu").manual_seed(5)
+        for k in ("d", "u", "v", "w", "T"):
+            # ghost layers start from distinct positive values (the caller's, where kept)
+            a = st.t[k]
+            ghost = torch.rand(a.shape, generator=gen, dtype=torch.float64).to(a.device) + 0.5
+            interior = st.interior(k).clone()
+            a.copy_(ghost)
+            st.interior(k).copy_(interior)
+        st.t["cc"].fill_(-1.0)
+        if fused:
+            W.check(L.ws_prepare_rhs_inputs(ctx.handle, ctypes.byref(st.ws), s))
+        else:
+            fl = (W.WsField * 5)(st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T)
+            W.check(L.ws_ghost_fill_periodic(ctx.handle, fl, 5, s))
+            W.check(L.ws_sound_speed(ctx.handle, st.ws.T, st.ws.cc,
+                                     W.job((0, 0, 0), (nx + 2 * LAP, ny + 2 * LAP, nz + 2 * LAP)), s))
+        _sync()
+        # cc compared over the ghosted box (the flat pass also fills the pitch padding)
+        runs.append({k: st.t[k].cpu().numpy()[..., :nx + 2 * LAP] for k in ("d", "u", "v", "w", "T", "cc")})
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
 def test_time_advance_stages_bitwise(orc, ws):
     import torch
     nx, ny, nz = 20, 12, 9

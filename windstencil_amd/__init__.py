@@ -84,6 +84,7 @@ _SIGS = {
                         [ctypes.POINTER(WsMetricFields), WsSoa, WsJob, _i32, ctypes.c_void_p]),
     "ws_sound_speed": (ctypes.c_int, [ctypes.c_void_p, WsField, WsField, WsJob, ctypes.c_void_p]),
     "ws_ghost_fill_periodic": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsField), _i32, ctypes.c_void_p]),
+    "ws_prepare_rhs_inputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p]),
     "ws_du_comput": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.POINTER(WsMetricFields),
                                     ctypes.c_void_p]),
     "ws_viscous_rhs": (ctypes.c_int, [ctypes.c_void_p] + [WsField] * 4 + [ctypes.POINTER(WsMetricFields), WsSoa,

@@ -214,10 +214,7 @@ int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s)
 
 int prepare_rhs_inputs(ws_ctx *c, const ws_state *st, hipStream_t s)
 {
-    const ws_field g[5] = {st->d, st->u, st->v, st->w, st->T};
-    int r = launch_ghost_fill(*c, g, 5, s);
-    if (!r) r = launch_sound_speed(*c, st->T, st->cc, ghosted_job(c), s);
-    return r;
+    return launch_prepare(*c, st->d, st->u, st->v, st->w, st->T, st->cc, s);
 }
 
 int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s)
@@ -453,13 +450,27 @@ int ws_ghost_fill_periodic(ws_ctx *c, const ws_field *fields, int32_t n, hipStre
     return launch_ghost_fill(*c, fields, n, s);
 }
 
+static int check_prims(const ws_ctx *c, const ws_state *st)
+{
+    int r;
+    if ((r = check_field(c, st->d, "d")) || (r = check_field(c, st->u, "u")) || (r = check_field(c, st->v, "v")) ||
+        (r = check_field(c, st->w, "w")) || (r = check_field(c, st->T, "T")) || (r = check_field(c, st->cc, "cc")))
+        return r;
+    return WS_OK;
+}
+
+int ws_prepare_rhs_inputs(ws_ctx *c, const ws_state *st, hipStream_t s)
+{
+    if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
+    int r = check_prims(c, st);
+    return r ? r : launch_prepare(*c, st->d, st->u, st->v, st->w, st->T, st->cc, s);
+}
+
 int ws_du_comput(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics, hipStream_t s)
 {
     if (!c || !st) return set_error(WS_ERR_INVALID, "null argument");
     int r;
-    const ws_field g[5] = {st->d, st->u, st->v, st->w, st->T};
-    if ((r = ws_ghost_fill_periodic(c, g, 5, s))) return r;
-    if ((r = launch_sound_speed(*c, st->T, st->cc, ghosted_job(c), s))) return r;
+    if ((r = ws_prepare_rhs_inputs(c, st, s))) return r;
     if ((r = ws_inviscid_rhs(c, st->d, st->u, st->v, st->w, st->cc, metrics, st->du, interior_job(c), 0, s))) return r;
     if (c->cfg.viscous) return launch_viscous(*c, st->u, st->v, st->w, st->T, metrics, st->du, s);
     return WS_OK;

@@ -69,6 +69,17 @@ __global__ void __launch_bounds__(NT) k_sound_speed(ws_field T, ws_field cc, int
     cc.ptr[fidx(cc.pitch, nyg, x, y, z)] = sqrt(T.ptr[fidx(T.pitch, nyg, x, y, z)]) / ama;
 }
 
+// Sound speed over a whole ghosted field treated as one flat array (pitch padding
+// included, which no kernel reads): cc = sqrt(T)/Ma, two points per thread.
+__global__ void __launch_bounds__(NT) k_sound_flat(const double2 *T, double2 *cc, long n2, double ama)
+{
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= n2) return;
+    const double2 t = T[i];
+    cc[i] = make_double2(sqrt(t.x) / ama, sqrt(t.y) / ama);
+}
+
 __global__ void __launch_bounds__(NT) k_cons_to_pri(KParams k, ws_soa f, ws_field d, ws_field u, ws_field v,
                                                     ws_field w, ws_field T, ws_field P)
 {
@@ -303,6 +314,24 @@ int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s)
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "ghost fill launch");
+}
+
+int launch_prepare(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T, ws_field cc,
+                   hipStream_t s)
+{
+    const KParams &k = c.kp;
+    const ws_field g[5] = {d, u, v, w, T};
+    int r = launch_ghost_fill(c, g, 5, s);
+    if (r) return r;
+    const ws_job box{{0, 0, 0}, {(uint32_t)(k.nx + 2 * LAP), (uint32_t)(k.ny + 2 * LAP), (uint32_t)(k.nz + 2 * LAP)}};
+    const long n = (long)T.pitch * (k.ny + 2 * LAP) * (k.nz + 2 * LAP);
+    // the flat form needs one layout and 16-byte alignment; otherwise the box kernel
+    if (T.pitch != cc.pitch || n % 2 || ((uintptr_t)T.ptr | (uintptr_t)cc.ptr) % 16)
+        return launch_sound_speed(c, T, cc, box, s);
+    hipLaunchKernelGGL(k_sound_flat, dim3(blocks_for(n / 2)), dim3(NT), 0, s, (const double2 *)T.ptr,
+                       (double2 *)cc.ptr, n / 2, k.ama);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "sound speed launch");
 }
 
 int launch_sound_speed(const Ctx &c, ws_field T, ws_field cc, ws_job job, hipStream_t s)
