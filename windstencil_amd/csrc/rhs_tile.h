@@ -344,14 +344,23 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     }
 }
 
+// Grid form: one work-group per tile.  The grid is 1-D with 8 * per work-groups and
+// work-group b runs tile (b % 8) * per + b / 8 (x tiles fastest): under the round-robin
+// dispatch of work-groups to the eight XCDs each XCD gets a contiguous range of tiles, so
+// neighbouring tiles, which share halo lines and the cache lines of their edge columns,
+// meet in the same L2.
 template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND>
 __global__ void __launch_bounds__(320)
 k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_metric_fields mf,
-           ws_soa du, Box box)
+           ws_soa du, Box box, int gx, int gy, int ntiles)
 {
     using TL = Tile<DIR, L, W, K>;
     __shared__ double sp[10 * TL::QS + TL::PAD];
-    const TileGeo g = tile_geo<DIR, L, W, K>(box, blockIdx.x, blockIdx.y, blockIdx.z);
+    const int per = (ntiles + 7) / 8;
+    const int tile = ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
+    if (tile >= ntiles) return;
+    const int r = tile / gx;
+    const TileGeo g = tile_geo<DIR, L, W, K>(box, tile - r * gx, r % gy, r / gy);
     TileLoads<DIR, L, W, K> ld;
     tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
     tile_compute<DIR, L, W, K, FIELDS, ACC, MIX, BND>(k, mf, du, g, ld, sp, [] {});
@@ -456,11 +465,15 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
 // state, so it needs no more registers than the single-box kernel.
 template <int DIR, int L, int W, int K, bool ACC, bool MIX>
 __global__ void __launch_bounds__(320)
-k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles)
+k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles)
 {
     using TL = Tile<DIR, L, W, K>;
     __shared__ double sp[10 * TL::QS + TL::PAD];
-    const int4 tl = tiles[blockIdx.x];
+    // XCD-contiguous ranges of the tile list, as in k_rhs_tile
+    const int per = (ntiles + 7) / 8;
+    const int tile = ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
+    if (tile >= ntiles) return;
+    const int4 tl = tiles[tile];
     const BatchBlock &b = blocks[tl.x];
     KParams kb = k;
     kb.nx = b.n[0]; kb.ny = b.n[1]; kb.nz = b.n[2];
@@ -515,13 +528,15 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     if (DIR == 0) grid = dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
     else if (DIR == 1) grid = dim3((nxa + W - 1) / W, (ny + L - 1) / L, nz);
     else grid = dim3((nxa + W - 1) / W, (nz + L - 1) / L, ny);
+    const int ntiles = (int)(grid.x * grid.y * grid.z);
+    const dim3 grid1(8 * ((ntiles + 7) / 8));
     if (k.bnd_any[DIR]) {
         // one-sided closures in this direction: the grid schedule with the closure code
-        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, true>), grid, dim3(TL::NT), 0, s, k, d, u, v,
-                           w, cc, mf, du, box);
+        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, true>), grid1, dim3(TL::NT), 0, s, k, d, u, v,
+                           w, cc, mf, du, box, (int)grid.x, (int)grid.y, ntiles);
     } else if (FIELDS || sched == WS_SCHED_GRID) {
-        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, false>), grid, dim3(TL::NT), 0, s, k, d, u, v,
-                           w, cc, mf, du, box);
+        hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, false>), grid1, dim3(TL::NT), 0, s, k, d, u,
+                           v, w, cc, mf, du, box, (int)grid.x, (int)grid.y, ntiles);
     } else if (sched == WS_SCHED_PREFETCH) {
         launch_persist<DIR, L, W, K, ACC, MIX, true>(k, d, u, v, w, cc, du, box, grid.x, grid.y, grid.z, s);
     } else {
@@ -566,8 +581,8 @@ void launch_batch_dir(const KParams &k, const BatchTiles &bt, hipStream_t s)
     const int n = bt.ntiles[DIR];
     if (n <= 0) return;
     if (bt.schedule == WS_SCHED_GRID) {
-        hipLaunchKernelGGL((k_rhs_batch_grid<DIR, L, W, K, ACC, MIX>), dim3(n), dim3(Tile<DIR, L, W, K>::NT), 0, s,
-                           k, bt.blocks, bt.tiles[DIR]);
+        hipLaunchKernelGGL((k_rhs_batch_grid<DIR, L, W, K, ACC, MIX>), dim3(8 * ((n + 7) / 8)),
+                           dim3(Tile<DIR, L, W, K>::NT), 0, s, k, bt.blocks, bt.tiles[DIR], n);
         return;
     }
     static const int resident = resident_of(k_rhs_batch<DIR, L, W, K, ACC, MIX, PF>, Tile<DIR, L, W, K>::NT);
