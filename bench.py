@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "inviscid RHS Mcell-updates/s, FP64 vs mixed at 600x256x128; % attainable roof"
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 ALGO_BYTES_PER_CELL = 80     # 5 FP64 state in + 5 FP64 RHS out
 # viscous kernels: u, v, w, T in (32 B) and twelve flux fields out (96 B); the twelve
@@ -66,6 +67,19 @@ def parse():
 # op-counting build; tests/test_bench_contract.py keeps the two in step.
 ALGO_FLOPS_PER_CELL = 7431
 ALGO_FLOPS_PER_DIRECTION = 2477
+# In the mixed RHS the WENO part (10 x 230 per direction) runs in FP32 and the split and
+# the flux difference (137 + 40) in FP64; its roof is the time of those FLOPs at the
+# two peaks: 2477 / (177 / 78.6 + 2300 / 157.3) = 146.8 TFLOP/s of algorithmic FLOPs.
+ALGO_FLOPS_FP64_PART = 137 + 40
+ALGO_FLOPS_FP32_PART = 10 * 230
+
+
+def peak_tflops(mixed):
+    """The VALU roof of the RHS for its precision (algorithmic FLOPs per second)."""
+    if not mixed:
+        return PEAK_FP64_TFLOPS
+    return ALGO_FLOPS_PER_DIRECTION / (ALGO_FLOPS_FP64_PART / PEAK_FP64_TFLOPS +
+                                       ALGO_FLOPS_FP32_PART / PEAK_FP32_TFLOPS)
 
 
 def algo_flops():
@@ -177,6 +191,7 @@ def bench_batched(args):
     kern_ms = e0.elapsed_time(e1) / args.steps
     flops_cell, _ = algo_flops()
     achieved = flops_cell * N / (kern_ms * 1e-3) / 1e12
+    mixed = prec == W.WS_PREC_MIXED
     nsteps = args.rk3_steps or 10
     W.check(L.ws_batch_rk3_run(b.handle, 1, 0.5, 0, sh))      # warm-up step
     t0 = time.perf_counter()
@@ -186,8 +201,8 @@ def bench_batched(args):
                   ms, "f64" if prec == W.WS_PREC_FP64 else "f64-state/mixed-rhs",
                   {"workload": "48 periodic blocks, extents U{32..256} (seed 1), Mach U[0.3,3] freestream + blobs",
                    "blocks": len(extents), "cells": N, "precision": args.precision},
-                  roofline={"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-                            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                  roofline={"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tflops(mixed), 2),
+                            "unit": "TFLOP/s", "frac": round(achieved / peak_tflops(mixed), 4), "traffic": None,
                             "kernel": "k_rhs_batch x3 directions (one RHS of every block)",
                             "kernel_ms": round(kern_ms, 4)},
                   tile_shape=chosen.value, schedule=sched.value, autotune_ms=tune,
@@ -238,12 +253,19 @@ def bench_rk3(args):
     f = st.interior("f")
     finite = bool(torch.isfinite(f).all())
     ms = secs / nsteps * 1e3
+    # the step against the RHS roof: the three RHS evaluations' algorithmic FLOPs over the
+    # whole step time (the memory-bound stage, ghost-fill and CFL kernels count as time)
+    mixed = prec == W.WS_PREC_MIXED
+    achieved = 3 * ALGO_FLOPS_PER_CELL * N / (ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tflops(mixed), 2),
+            "unit": "TFLOP/s", "frac": round(achieved / peak_tflops(mixed), 4), "traffic": None,
+            "kernel": "whole RK3 step (3 RHS of algorithmic FLOPs over the step time)"}
     out = _common(args, f"SSP-RK3 Mcell-updates/s (N x 3 stages x steps / time), {nx}x{ny}x{nz}",
                   N * 3 * nsteps / secs / 1e6, ms, "f64" if prec == W.WS_PREC_FP64 else "f64-state/mixed-rhs",
                   {"workload": f"{nsteps} SSP-RK3 steps of the {nx}x{ny}x{nz} periodic box ({what}), CFL 0.5, "
                                "one hipGraph per step", "grid": [nx, ny, nz], "precision": args.precision,
                    "cells": N},
-                  rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
+                  roofline=roof, rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
                   totals=[float(v) for v in f.sum(dim=(1, 2, 3)).cpu()])
     out["steps"] = nsteps
     print(json.dumps(out))
@@ -333,10 +355,15 @@ def main():
         # the other precision on the same state, for the FP64-vs-mixed comparison
         ms2, _, kern2, shape2, tune2 = run(other)
         du2 = st.interior("du").clone()
+        ach2 = ALGO_FLOPS_PER_CELL * N / (kern2 * 1e-3) / 1e12
+        peak2 = peak_tflops(other == W.WS_PREC_MIXED)
         second = {"precision": "mixed" if other == W.WS_PREC_MIXED else "fp64",
                   "value": round(N / (ms2 * 1e-3) / 1e6, 2), "ms_per_step": round(ms2, 4),
-                  "kernel_ms": round(kern2, 4), "tile_shape/schedule": shape2, "autotune_ms": tune2}
+                  "kernel_ms": round(kern2, 4), "tile_shape/schedule": shape2, "autotune_ms": tune2,
+                  "roofline": {"bound": "valu", "achieved": round(ach2, 3), "peak": round(peak2, 2),
+                               "unit": "TFLOP/s", "frac": round(ach2 / peak2, 4)}}
     ms, prep_ms, kern_ms, shape, tune = run(prec)
+    mixed = prec == W.WS_PREC_MIXED
     value = N / (ms * 1e-3) / 1e6
     flops_cell, flops_dir = algo_flops()
     achieved = flops_cell * N / (kern_ms * 1e-3) / 1e12
@@ -425,8 +452,8 @@ def main():
         "config": {"workload": f"one inviscid RHS of the {nx}x{ny}x{nz} periodic box "
                                "(ghost fill + sound speed + WENO7-SYMBO/Steger-Warming in x, y, z)",
                    "grid": [nx, ny, nz], "precision": args.precision, "cells": N},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tflops(mixed), 2),
+                     "unit": "TFLOP/s", "frac": round(achieved / peak_tflops(mixed), 4), "traffic": traffic,
                      "kernel": "k_rhs_tile x3 directions (one RHS)",
                      "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
         "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
