@@ -17,6 +17,7 @@ import ctypes
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -46,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline leg")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-viscous", action="store_true", help="skip the Navier-Stokes (viscous) leg")
+    ap.add_argument("--no-steps200", action="store_true",
+                    help="skip the 200-step FP64-vs-mixed state comparison of the default run")
     ap.add_argument("--no-second", action="store_true",
                     help="skip timing the other precision (mixed when --precision fp64)")
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape (default: autotune)")
@@ -120,10 +123,69 @@ def cpu_baseline(prims, nx, ny, nz, budget_s):
         if time.perf_counter() > t_end or len(times) >= 5:
             break
     med = statistics.median(times)
-    return {"value": round(nx * ny * nz / med / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{len(times)} full RHS evaluations of the {nx}x{ny}x{nz} box (median), "
-                      f"C restatement -O3 -fopenmp, {threads} threads"}
+    out = {"value": round(nx * ny * nz / med / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{len(times)} full RHS evaluations of the {nx}x{ny}x{nz} box (median), "
+                     f"C restatement -O3 -fopenmp, {threads} threads"}
+    out.update(cpu_config1(threads))
+    return out
+
+
+def _host_info():
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_config1(threads):
+    """The CPU-path workload in full (64x32x16 Mach 2 freestream with 8 blobs: one RHS,
+    then 10 SSP-RK3 steps at CFL 0.5) on `threads` host threads, and its RHS on one
+    thread; medians of 5 runs after a warm-up."""
+    from oracle import oracle as O
+    from tests.helpers import ghosted
+    from windstencil_amd import initial as I
+    nx, ny, nz = 64, 32, 16
+    prims = I.freestream_blobs(nx, ny, nz, 1)
+
+    def rhs_rate(nthreads):
+        p = O.params(nx, ny, nz, nthreads=nthreads)
+        g = [ghosted(a) for a in prims]
+        for a in g:
+            O.ghost_fill_periodic(p, a)
+        cc = O.sound_speed(p, g[4])
+        met, aj = O.metrics_cartesian(p), np.ones_like(cc)
+        times = []
+        for k in range(6):
+            du = np.zeros((5, nz, ny, nx))
+            t0 = time.perf_counter()
+            O.inviscid_rhs(p, g[0], g[1], g[2], g[3], cc, met, aj, du)
+            if k:
+                times.append(time.perf_counter() - t0)
+        return nx * ny * nz / statistics.median(times) / 1e6
+
+    def run_full():
+        p = O.params(nx, ny, nz, nthreads=threads)
+        g = [ghosted(a) for a in prims]
+        t0 = time.perf_counter()
+        f = O.pri_to_cons(p, *g)
+        O.du_comput(p, *g)
+        O.rk3_run(p, 10, -0.5, f, *g)
+        return time.perf_counter() - t0
+
+    run_full()
+    full = statistics.median(run_full() for _ in range(5))
+    return {"config1": {"workload": "64x32x16 Mach 2 + 8 blobs: one RHS + 10 SSP-RK3 steps (CFL 0.5)",
+                        "seconds": round(full, 4), "threads": threads,
+                        "rhs_mcell_per_s": round(rhs_rate(threads), 3),
+                        "rhs_mcell_per_s_1_thread": round(rhs_rate(1), 3)},
+            "host": _host_info()}
 
 
 def _common(args, metric, value, ms, dtype, config, **extra):
@@ -389,6 +451,34 @@ def main():
         W.check(L.ws_ctx_set_kernel(ctx.handle, W.WS_KERNEL_TILED))
         del du_fast
 
+    steps200 = None
+    if not args.no_second and not args.no_steps200:
+        # the mixed form's accuracy over a run: 200 SSP-RK3 steps (CFL 0.5 from the device
+        # each step, graph-replayed) of the same initial state in both precisions; the
+        # conserved states compared (relative L2; Linf relative to max |f|)
+        def rk3_state(precision):
+            c2 = W.Context(W.default_config(nx, ny, nz, precision=precision))
+            s2 = W.TorchState(nx, ny, nz)
+            s2.set_prims(*prims)
+            W.check(L.ws_pri_to_cons(c2.handle, s2.ws.f, s2.ws.d, s2.ws.u, s2.ws.v, s2.ws.w, s2.ws.T,
+                                     W.job((0, 0, 0), (nx, ny, nz)), sh))
+            W.check(L.ws_ctx_set_tile_shape(c2.handle, int(shape.split("/")[0])))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            W.check(L.ws_rk3_run(c2.handle, ctypes.byref(s2.ws), 200, 0.5, 1, sh))
+            secs = time.perf_counter() - t0
+            s2.sync_from_ws()
+            f = s2.interior("f").clone()
+            del s2, c2
+            return f, secs
+        fa, sa = rk3_state(W.WS_PREC_FP64)
+        fb, sb = rk3_state(W.WS_PREC_MIXED)
+        steps200 = {"steps": 200, "cfl": 0.5,
+                    "state_rel_l2": float(torch.linalg.vector_norm(fb - fa) / torch.linalg.vector_norm(fa)),
+                    "state_linf_over_max": float((fb - fa).abs().max() / fa.abs().max()),
+                    "seconds_fp64": round(sa, 3), "seconds_mixed": round(sb, 3)}
+        del fa, fb
+
     viscous = None
     if not args.no_viscous:
         # the Navier-Stokes RHS: the same inviscid RHS plus the viscous terms (CD6,
@@ -461,6 +551,7 @@ def main():
         "autotune_key": "tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
         "parity_rel_l2_vs_anchor": parity,
         "other_precision": second,
+        "mixed_vs_fp64_200_rk3_steps": steps200,
         "navier_stokes": viscous,
     }
     if not args.no_cpu:
