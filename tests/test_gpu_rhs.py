@@ -49,6 +49,25 @@ def test_tiled_fp64_matches_oracle(orc, ws, shape, kind, tile):
     assert err <= TOL_FP64
 
 
+@pytest.mark.parametrize("shape", [(4, 4, 4), (4, 9, 5), (7, 4, 6), (5, 6, 4), (129, 65, 33)])
+def test_smallest_and_ragged_extents(orc, ws, shape):
+    """Extents at the minimum the periodic halo allows (4 cells, where the halo is the
+    whole interior) and ragged sizes far from any tile multiple: anchor bitwise, tiled
+    FP64 within the drop-in bar for every tile shape, mixed within 1e-6."""
+    nx, ny, nz = shape
+    prims = random_state(nx, ny, nz, 17, "rough")
+    ref = oracle_du(orc, prims)
+    assert np.array_equal(gpu_du_comput(make_ctx(nx, ny, nz, kernel=W.WS_KERNEL_ANCHOR), upload_state(prims)), ref)
+    for tile in W.TILE_SHAPES:
+        ctx = make_ctx(nx, ny, nz)
+        ctx.set_tile_shape(tile)
+        err = rel_l2(gpu_du_comput(ctx, upload_state(prims)), ref)
+        log(f"  {shape} tile {tile}: rel L2 {err:.3e}")
+        assert err <= TOL_FP64
+    err = rel_l2(gpu_du_comput(make_ctx(nx, ny, nz, precision=W.WS_PREC_MIXED), upload_state(prims)), ref)
+    assert err <= 1e-6
+
+
 def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
     """All tile shapes give the same RHS to rounding; the autotuned context still
     matches the oracle and remembers its choice for the grid."""

@@ -60,6 +60,20 @@ def test_viscous_terms(orc, ws, order, kind, kernel):
         assert err <= 1e-13
 
 
+@pytest.mark.parametrize("shape", [(4, 4, 4), (5, 9, 4), (37, 6, 41)])
+def test_viscous_terms_small_and_ragged_extents(orc, ws, shape):
+    """Extents at the halo minimum and ragged ones (partial column tiles, z chunks
+    shorter than a march): the anchor family bit for bit, the tiled family within 1e-13."""
+    nx, ny, nz = shape
+    prims = random_state(nx, ny, nz, 21, "rough")
+    ref = _oracle_full(orc, prims, 8)
+    got = gpu_du_comput(make_ctx(nx, ny, nz, kernel=W.WS_KERNEL_ANCHOR, **_vis_cfg(8)), upload_state(prims))
+    assert np.array_equal(got, ref)
+    err = rel_l2(gpu_du_comput(make_ctx(nx, ny, nz, **_vis_cfg(8)), upload_state(prims)), ref)
+    log(f"  {shape}: tiled + viscous rel L2 {err:.3e}")
+    assert err <= 1e-12
+
+
 @pytest.mark.parametrize("order", [6, 8])
 def test_full_rhs_with_viscous_terms(orc, ws, order):
     nx, ny, nz = 40, 24, 20
