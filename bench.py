@@ -199,8 +199,8 @@ def _common(args, metric, value, ms, dtype, config, **extra):
 def bench_batched(args):
     """48 periodic blocks with extents drawn from U{32..256} (seed 1), each a freestream
     of Mach U[0.3, 3] in a seeded direction with Gaussian blobs.  A step is one RHS of
-    every block (per-block ghost fill and sound speed, then the batched persistent RHS
-    launches); then RK3 steps of the whole batch replayed from one hipGraph per step."""
+    every block (per-block ghost fill and sound speed, then the batched RHS launches, one
+    per direction); then RK3 steps of the whole batch replayed from one hipGraph per step."""
     import torch
     import windstencil_amd as W
     from windstencil_amd import initial as I

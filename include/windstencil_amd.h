@@ -298,7 +298,7 @@ int ws_load_jacobi(ws_ctx *ctx, const char *path, const ws_metric_fields *metric
  * decompositions of one grid).  Every block has the batch's physics, metrics and
  * precision, its own extents, spacing h = 1/(n-1) per direction (the reference's
  * convention, src/runtime/OCFD_init.cpp) and a caller-owned state.  The RHS of all
- * blocks is one persistent launch per direction over a tile list; one dt (the minimum
+ * blocks is one launch per direction over a tile list; one dt (the minimum
  * of the blocks' CFL steps) advances every block. */
 typedef struct ws_batch ws_batch;
 

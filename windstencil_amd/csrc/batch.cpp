@@ -1,9 +1,10 @@
 // batch.cpp -- many periodic blocks of uneven size advanced together.
 //
 // Each block keeps its own context (extents, spacing 1/(n-1)) and its caller-owned
-// state.  The inviscid RHS of all blocks runs as one persistent launch per direction
-// over a device tile list (block, tile coordinates) built here for every tile shape;
-// the ghost fill, sound speed, stage update and primitive recovery run per block.  A
+// state.  The inviscid RHS of all blocks runs as one launch per direction over a device
+// tile list (block, tile coordinates) built here for every tile shape: one work-group
+// per tile, or a persistent grid walking contiguous runs of the list (the batch's
+// schedule); the ghost fill, sound speed and fused RK3 stage run per block.  A
 // batch step uses one dt for every block: the minimum of the blocks' CFL steps.  A
 // whole SSP-RK3 step (CFL reduction included) can be captured and replayed as one
 // hipGraph.
