@@ -292,6 +292,27 @@ int ws_jacobi_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, do
 int ws_load_jacobi(ws_ctx *ctx, const char *path, const ws_metric_fields *metrics, const ws_field *coords,
                    hipStream_t stream);
 
+/* OCFD3d-Mesh.dat: the mesh coordinates x, y, z, each as NZ plane records (the layout of
+ * the Jacobi file's first three fields), no header; read by Init_Jacobian3d when no Jacobi
+ * file exists (src/solver/OCFD_Comput_Jacobian3d.cpp:120-135). */
+int ws_mesh_write_host(const char *path, int32_t nx, int32_t ny, int32_t nz, const double *const *xyz);
+int ws_mesh_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, double *const *xyz);
+/* Metric terms from ghosted mesh coordinates coords[0..2] (their ghost layers as the
+ * caller left them): metrics->A[0..8] = Akx..Asz, the cofactors of d(x,y,z)/d(i,j,k),
+ * and metrics->ajac = 1/det, from sixth-order central differences divided by the spacing;
+ * jacbound[d] = 1 (the reference's Jacbound, at least 8 cells) uses the one-sided and
+ * low-order forms on the four cells next to each face of direction d.  Face ghosts repeat
+ * the adjacent cell; periodic directions then get the periodic face halo.  Replaces
+ * Comput_Jacobian3d / comput_Jac3d (src/solver/OCFD_Comput_Jacobian3d.cpp:215-429).
+ * jacbound NULL means all 0. */
+int ws_comput_jacobian(ws_ctx *ctx, const ws_field *coords, const ws_metric_fields *metrics,
+                       const int32_t *jacbound, hipStream_t stream);
+/* Init_Jacobian3d's mesh branch: every coordinate and metric value set to 1.0, the mesh
+ * file read into the interior of coords, the coordinates' periodic face halo filled, then
+ * ws_comput_jacobian (src/solver/OCFD_Comput_Jacobian3d.cpp:33-145). */
+int ws_load_mesh(ws_ctx *ctx, const char *path, const ws_field *coords, const ws_metric_fields *metrics,
+                 const int32_t *jacbound, hipStream_t stream);
+
 /* ---- batches of blocks ------------------------------------------------------
  * Many independent periodic blocks of uneven size advanced together.  No counterpart
  * in the reference, which runs one block per process (its multi-block runs are MPI

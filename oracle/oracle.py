@@ -179,6 +179,18 @@ def amu(p, T, re, ref_t):
     return out
 
 
+def comput_jacobian(p, x, y, z, jacbound=(0, 0, 0)):
+    """Metric terms from ghosted mesh coordinates (their face ghosts as the caller's
+    exchange left them): [Akx, Aky, Akz, Aix, Aiy, Aiz, Asx, Asy, Asz, Ajac], ghosted,
+    each initialised to 1.0 where the computation leaves a value untouched (edges and
+    corners), as the reference's Init_Jacobian3d does."""
+    out = [np.ones_like(x) for _ in range(10)]
+    xyz = (_dp * 3)(_ptr(x), _ptr(y), _ptr(z))
+    outp = (_dp * 10)(*[_ptr(a) for a in out])
+    lib().orc_comput_jacobian(ctypes.byref(p), xyz, (ctypes.c_int * 3)(*jacbound), outp)
+    return out
+
+
 def viscous_rhs(p, u, v, w, T, re, pr, ref_t, order=6, metrics=None, ajac=None, du=None):
     """Viscous terms added to du[1..4] (a zero du unless given) from ghosted u, v, w, T
     whose periodic ghosts are filled."""

@@ -678,3 +678,80 @@ void orc_viscous_rhs(const orc_params *p, double re, double pr, double ref_t, in
     free(amu);
     free(ev);
 }
+
+/* ---- metrics from mesh coordinates (src/solver/OCFD_Comput_Jacobian3d.cpp:215-429) ---------- */
+
+/* Derivative of ghosted field f along direction dir, interior cell (i, j, k), divided by the
+ * direction's spacing: the sixth-order central form OCFD_dx0_CD6_kernel
+ * (src/kernels/OCFD_Schemes.cpp:1229-1297); with bound set, the four cells next to each
+ * face use the one-sided/low-order forms of OCFD_Dx0_bound_kernel_m/_p
+ * (src/boundary/OCFD_bound_Scheme.cpp:268-307, the y and z copies to :390), selected per
+ * direction by Jacbound (OCFD_dx0_jac, src/solver/OCFD_Schemes_Choose.cpp:655-700). */
+static double metric_deriv(const orc_params *p, const double *f, int dir, int bound, long i, long j, long k)
+{
+    const long n = dir == 0 ? p->nx : dir == 1 ? p->ny : p->nz;
+    const long c = dir == 0 ? i : dir == 1 ? j : k;
+    const double h = dir == 0 ? p->hx : dir == 1 ? p->hy : p->hz;
+#define FV(o) f[G(p, dir == 0 ? (o) : i, dir == 1 ? (o) : j, dir == 2 ? (o) : k)]
+    if (bound) {
+        const long t = n - 1;
+        if (c == 0) return (FV(1) - FV(0)) / h;
+        if (c == 1) return (FV(2) - FV(0)) * 0.5 / h;
+        if (c == 2) return (FV(0) - 8.0 * FV(1) + 8.0 * FV(3) - FV(4)) / (12.0 * h);
+        if (c == 3) return (FV(6) - FV(0) - 9.0 * (FV(5) - FV(1)) + 45.0 * (FV(4) - FV(2))) / (60.0 * h);
+        if (c == t) return (FV(t) - FV(t - 1)) / h;
+        if (c == t - 1) return (FV(t) - FV(t - 2)) * 0.5 / h;
+        if (c == t - 2) return (FV(t - 4) - 8.0 * FV(t - 3) + 8.0 * FV(t - 1) - FV(t)) / (12.0 * h);
+        if (c == t - 3) return (FV(t) - FV(t - 6) - 9.0 * (FV(t - 1) - FV(t - 5)) + 45.0 * (FV(t - 2) - FV(t - 4))) / (60.0 * h);
+    }
+    return (45.0 * (FV(c + 1) - FV(c - 1)) - 9.0 * (FV(c + 2) - FV(c - 2)) + (FV(c + 3) - FV(c - 3))) / (60.0 * h);
+#undef FV
+}
+
+/* Metric terms from ghosted mesh coordinates xyz[0..2] (face ghosts filled as the caller's
+ * exchange left them): out[0..8] = Akx, Aky, Akz, Aix, Aiy, Aiz, Asx, Asy, Asz and
+ * out[9] = Ajac (ghosted fields).  Interior cells take the cofactors of d(x,y,z)/d(i,j,k)
+ * and Ajac = 1/det (comput_Jac3d_kernal); each face's ghost layers copy the adjacent
+ * interior cell; then the periodic face halo overwrites the periodic directions
+ * (exchange_boundary_xyz_packed_dev in Comput_Jacobian3d). */
+void orc_comput_jacobian(const orc_params *p, const double *const *xyz, const int jacbound[3], double *const *out)
+{
+    const long nx = p->nx, ny = p->ny, nz = p->nz;
+    long i, j, k;
+    int m, q;
+#pragma omp parallel for private(j, i, m, q) collapse(2)
+    for (k = 0; k < nz; k++)
+        for (j = 0; j < ny; j++)
+            for (i = 0; i < nx; i++) {
+                double dv[3][3];   /* dv[c][d]: derivative of coordinate c along direction d */
+                for (m = 0; m < 3; m++)
+                    for (q = 0; q < 3; q++) dv[m][q] = metric_deriv(p, xyz[m], q, jacbound[q], i, j, k);
+                const double xi1 = dv[0][0], xj1 = dv[0][1], xk1 = dv[0][2];
+                const double yi1 = dv[1][0], yj1 = dv[1][1], yk1 = dv[1][2];
+                const double zi1 = dv[2][0], zj1 = dv[2][1], zk1 = dv[2][2];
+                double v[10];
+                v[9] = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
+                              xi1 * zj1 * yk1);
+                v[0] = (yj1 * zk1 - zj1 * yk1);
+                v[1] = (zj1 * xk1 - xj1 * zk1);
+                v[2] = (xj1 * yk1 - yj1 * xk1);
+                v[3] = (yk1 * zi1 - zk1 * yi1);
+                v[4] = (zk1 * xi1 - xk1 * zi1);
+                v[5] = (xk1 * yi1 - yk1 * xi1);
+                v[6] = (yi1 * zj1 - zi1 * yj1);
+                v[7] = (zi1 * xj1 - xi1 * zj1);
+                v[8] = (xi1 * yj1 - yi1 * xj1);
+                for (m = 0; m < 10; m++) {
+                    double *o = out[m];
+                    long g;
+                    o[G(p, i, j, k)] = v[m];
+                    if (i == 0) for (g = 1; g <= LAP; g++) o[G(p, -g, j, k)] = v[m];
+                    if (i == nx - 1) for (g = 1; g <= LAP; g++) o[G(p, nx - 1 + g, j, k)] = v[m];
+                    if (j == 0) for (g = 1; g <= LAP; g++) o[G(p, i, -g, k)] = v[m];
+                    if (j == ny - 1) for (g = 1; g <= LAP; g++) o[G(p, i, ny - 1 + g, k)] = v[m];
+                    if (k == 0) for (g = 1; g <= LAP; g++) o[G(p, i, j, -g)] = v[m];
+                    if (k == nz - 1) for (g = 1; g <= LAP; g++) o[G(p, i, j, nz - 1 + g)] = v[m];
+                }
+            }
+    for (m = 0; m < 10; m++) orc_ghost_fill_periodic(p, out[m]);
+}

@@ -121,6 +121,11 @@ int orc_closure_m(const double *s, long n, long f, int lo, int hi, double *out);
  * division by 12). */
 void orc_weno7_symbo_p_parts(const double *s8, double S[5], double wts[5], double q[5]);
 
+/* Metric terms Akx..Asz (out[0..8]) and Ajac (out[9]) from ghosted mesh coordinates
+ * (comput_Jac3d, src/solver/OCFD_Comput_Jacobian3d.cpp:215-429); jacbound[d] = 1 selects
+ * the one-sided derivative forms at the faces of direction d (the reference's Jacbound). */
+void orc_comput_jacobian(const orc_params *p, const double *const *xyz, const int jacbound[3], double *const *out);
+
 #ifdef __cplusplus
 }
 #endif

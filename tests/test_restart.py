@@ -116,3 +116,24 @@ def test_jacobi_file_round_trip(tmp_path):
     W.check(L.ws_restart_write_host(r5, nx, ny, nz, 1, 0.0, _ptrs(f[:5])))
     assert L.ws_jacobi_read_host(r5, nx, ny, nz, (ctypes.c_void_p * 13)(*[a.ctypes.data for a in out])) \
         == W.WS_ERR_INVALID
+
+
+def test_mesh_file_round_trip(tmp_path):
+    """OCFD3d-Mesh.dat: x, y, z as plane records, no header (read with the Jacobi file's
+    offsets, src/solver/OCFD_Comput_Jacobian3d.cpp:120-135); the first three fields of a
+    Jacobi file have the same bytes."""
+    nx, ny, nz = 6, 5, 4
+    rng = np.random.default_rng(11)
+    f = [np.ascontiguousarray(rng.standard_normal((nz, ny, nx))) for _ in range(3)]
+    path = str(tmp_path / "OCFD3d-Mesh.dat").encode()
+    L = W.lib()
+    W.check(L.ws_mesh_write_host(path, nx, ny, nz, (ctypes.c_void_p * 3)(*[a.ctypes.data for a in f])))
+    data = open(path, "rb").read()
+    mark = np.array([nx * ny * 8], dtype=np.int32).tobytes()
+    assert data == b"".join(mark + a[z].tobytes() + mark for a in f for z in range(nz))
+    out = [np.zeros((nz, ny, nx)) for _ in range(3)]
+    W.check(L.ws_mesh_read_host(path, nx, ny, nz, (ctypes.c_void_p * 3)(*[a.ctypes.data for a in out])))
+    assert all(np.array_equal(a, b) for a, b in zip(out, f))
+    # another grid's mesh file is rejected
+    assert L.ws_mesh_read_host(path, nx, ny, nz - 1, (ctypes.c_void_p * 3)(*[a.ctypes.data for a in out])) \
+        == W.WS_ERR_INVALID

@@ -85,6 +85,12 @@ _SIGS = {
     "ws_sound_speed": (ctypes.c_int, [ctypes.c_void_p, WsField, WsField, WsJob, ctypes.c_void_p]),
     "ws_ghost_fill_periodic": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsField), _i32, ctypes.c_void_p]),
     "ws_prepare_rhs_inputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p]),
+    "ws_mesh_write_host": (ctypes.c_int, [ctypes.c_char_p, _i32, _i32, _i32, ctypes.POINTER(ctypes.c_void_p)]),
+    "ws_mesh_read_host": (ctypes.c_int, [ctypes.c_char_p, _i32, _i32, _i32, ctypes.POINTER(ctypes.c_void_p)]),
+    "ws_comput_jacobian": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsField), ctypes.POINTER(WsMetricFields),
+                                          ctypes.POINTER(_i32), ctypes.c_void_p]),
+    "ws_load_mesh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(WsField),
+                                    ctypes.POINTER(WsMetricFields), ctypes.POINTER(_i32), ctypes.c_void_p]),
     "ws_du_comput": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.POINTER(WsMetricFields),
                                     ctypes.c_void_p]),
     "ws_viscous_rhs": (ctypes.c_int, [ctypes.c_void_p] + [WsField] * 4 + [ctypes.POINTER(WsMetricFields), WsSoa,

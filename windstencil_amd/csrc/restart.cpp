@@ -9,7 +9,8 @@
 // i.e. Fortran sequential unformatted records.  Only the interior is stored.
 // OCFD3d-Jacobi.dat (read by Init_Jacobian3d, src/solver/OCFD_Comput_Jacobian3d.cpp:138-205)
 // has no header: x, y, z, Akx, Aky, Akz, Aix, Aiy, Aiz, Asx, Asy, Asz, Ajac as NZ plane
-// records each.
+// records each; OCFD3d-Mesh.dat (read there when no Jacobi file exists, :120-135) holds only
+// x, y, z, the same way.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -246,6 +247,89 @@ int ws_load_jacobi(ws_ctx *c, const char *path, const ws_metric_fields *mf, cons
     if ((r = launch_ghost_fill(*c, dst + 3, 10, s))) return r;
     if (coords && (r = launch_ghost_fill(*c, dst, 3, s))) return r;
     return WS_OK;
+}
+
+int ws_mesh_write_host(const char *path, int32_t nx, int32_t ny, int32_t nz, const double *const *xyz)
+{
+    if (!path || !xyz || !xyz[0] || !xyz[1] || !xyz[2]) return set_error(WS_ERR_INVALID, "mesh write: null argument");
+    int r = check_extents(nx, ny, nz);
+    if (r) return r;
+    File fh;
+    fh.f = std::fopen(path, "wb");
+    if (!fh.f) return set_error(WS_ERR_INVALID, std::string("mesh write: cannot open ") + path);
+    if (!write_planes(fh.f, nx, ny, nz, 3, xyz))
+        return set_error(WS_ERR_INVALID, std::string("mesh write: short write to ") + path);
+    if (std::fclose(fh.f) != 0) { fh.f = nullptr; return set_error(WS_ERR_INVALID, "mesh write: close failed"); }
+    fh.f = nullptr;
+    return WS_OK;
+}
+
+int ws_mesh_read_host(const char *path, int32_t nx, int32_t ny, int32_t nz, double *const *xyz)
+{
+    if (!path || !xyz || !xyz[0] || !xyz[1] || !xyz[2]) return set_error(WS_ERR_INVALID, "mesh read: null argument");
+    int r = check_extents(nx, ny, nz);
+    if (r) return r;
+    File fh;
+    fh.f = std::fopen(path, "rb");
+    if (!fh.f) return set_error(WS_ERR_INVALID, std::string("mesh read: cannot open ") + path);
+    if ((r = read_planes(fh.f, nx, ny, nz, 3, xyz))) return set_error(r, std::string("mesh read: ") + ws_last_error());
+    return WS_OK;
+}
+
+static int check_metric_args(const ws_ctx *c, const ws_field *coords, const ws_metric_fields *mf,
+                             const int32_t *jacbound)
+{
+    if (!c || !coords || !mf) return set_error(WS_ERR_INVALID, "metrics: null argument");
+    const uint32_t need = (uint32_t)(c->cfg.nx + 2 * LAP);
+    for (int q = 0; q < 3; ++q)
+        if (!coords[q].ptr || coords[q].pitch < need) return set_error(WS_ERR_INVALID, "metrics: bad coordinate field");
+    for (int q = 0; q < 10; ++q) {
+        const ws_field &f = q < 9 ? mf->A[q] : mf->ajac;
+        if (!f.ptr || f.pitch < need) return set_error(WS_ERR_INVALID, "metrics: bad metric field");
+    }
+    const int n[3] = {c->cfg.nx, c->cfg.ny, c->cfg.nz};
+    for (int d = 0; d < 3 && jacbound; ++d) {
+        if (jacbound[d] != 0 && jacbound[d] != 1) return set_error(WS_ERR_INVALID, "metrics: jacbound entries must be 0 or 1");
+        if (jacbound[d] && n[d] < 8)
+            return set_error(WS_ERR_INVALID, "metrics: one-sided derivative forms need at least 8 cells");
+    }
+    return WS_OK;
+}
+
+int ws_comput_jacobian(ws_ctx *c, const ws_field *coords, const ws_metric_fields *mf, const int32_t *jacbound,
+                       hipStream_t s)
+{
+    int r = check_metric_args(c, coords, mf, jacbound);
+    return r ? r : launch_comput_jacobian(*c, coords, *mf, jacbound, s);
+}
+
+int ws_load_mesh(ws_ctx *c, const char *path, const ws_field *coords, const ws_metric_fields *mf,
+                 const int32_t *jacbound, hipStream_t s)
+{
+    int r = check_metric_args(c, coords, mf, jacbound);
+    if (r) return r;
+    if (!path) return set_error(WS_ERR_INVALID, "mesh load: null path");
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<double> host(3 * n);
+    double *p[3] = {host.data(), host.data() + n, host.data() + 2 * n};
+    if ((r = ws_mesh_read_host(path, nx, ny, nz, p))) return r;
+    // Init_Jacobian3d: every coordinate and metric value starts at 1.0 (ghosts included)
+    ws_field all[13];
+    for (int q = 0; q < 3; ++q) all[q] = coords[q];
+    for (int q = 0; q < 9; ++q) all[3 + q] = mf->A[q];
+    all[12] = mf->ajac;
+    for (const ws_field &f : all)
+        if ((r = launch_fill(1.0, f.ptr, f.pitch, (uint32_t)(nx + 2 * LAP), (uint32_t)(ny + 2 * LAP),
+                             (uint32_t)(nz + 2 * LAP), s)))
+            return r;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(e, "mesh load: stream sync");
+    for (int q = 0; q < 3; ++q)
+        if ((r = host_to_field(p[q], nx, ny, nz, coords[q]))) return r;
+    // the coordinates' periodic face halo (exchange_boundary_xyz after the read), then the metrics
+    if ((r = launch_ghost_fill(*c, coords, 3, s))) return r;
+    return launch_comput_jacobian(*c, coords, *mf, jacbound, s);
 }
 
 }  // extern "C"

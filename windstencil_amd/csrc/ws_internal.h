@@ -115,6 +115,9 @@ int launch_viscous(Ctx &c, ws_field u, ws_field v, ws_field w, ws_field T, const
                    hipStream_t s);
 
 int launch_sound_speed(const Ctx &c, ws_field T, ws_field cc, ws_job job, hipStream_t s);
+// metric terms Akx..Asz and Ajac from ghosted mesh coordinates (k_metrics.hip)
+int launch_comput_jacobian(const Ctx &c, const ws_field *coords, const ws_metric_fields &mf, const int32_t *jacbound,
+                           hipStream_t s);
 // ghost fill of d, u, v, w, T, then the sound speed over the ghosted box
 int launch_prepare(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T, ws_field cc,
                    hipStream_t s);
