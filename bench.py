@@ -85,6 +85,23 @@ def peak_tflops(mixed):
                                        ALGO_FLOPS_FP32_PART / PEAK_FP32_TFLOPS)
 
 
+def measured_peaks(L, sh):
+    """The device's vector ALU throughput (ws_probe_valu): FMA, multiply and add in FP64
+    and scalar FP32 (tera lane-operations/s), and the attainable roofs of the two RHS forms
+    from the FMA rates (2 FLOP per FMA)."""
+    import windstencil_amd as W
+    rate = {}
+    for pname, prec in (("fp64", W.WS_PREC_FP64), ("fp32", W.WS_PREC_MIXED)):
+        for oname, op in (("fma", W.WS_PROBE_FMA), ("mul", W.WS_PROBE_MUL), ("add", W.WS_PROBE_ADD)):
+            v = ctypes.c_double()
+            W.check(L.ws_probe_valu(prec, op, ctypes.byref(v), sh))
+            rate[f"{pname}_{oname}"] = round(v.value, 2)
+    f64, f32 = 2 * rate["fp64_fma"], 2 * rate["fp32_fma"]
+    mixed_roof = ALGO_FLOPS_PER_DIRECTION / (ALGO_FLOPS_FP64_PART / f64 + ALGO_FLOPS_FP32_PART / f32)
+    return {"tera_lane_ops_per_s": rate, "fp64_fma_tflops": round(f64, 2), "fp32_fma_tflops": round(f32, 2),
+            "fp64_roof": round(f64, 2), "mixed_roof": round(mixed_roof, 2)}
+
+
 def algo_flops():
     return ALGO_FLOPS_PER_CELL, ALGO_FLOPS_PER_DIRECTION
 
@@ -429,6 +446,13 @@ def main():
     value = N / (ms * 1e-3) / 1e6
     flops_cell, flops_dir = algo_flops()
     achieved = flops_cell * N / (kern_ms * 1e-3) / 1e12
+    # the attainable roof on this device (FMA probe), beside the published peak
+    probe = measured_peaks(L, sh)
+    roof_key = "mixed_roof" if mixed else "fp64_roof"
+    if second is not None:
+        rk = "mixed_roof" if second["precision"] == "mixed" else "fp64_roof"
+        second["roofline"]["measured_roof"] = probe[rk]
+        second["roofline"]["frac_of_measured"] = round(second["roofline"]["achieved"] / probe[rk], 4)
     if second is not None:
         du1 = st.interior("du")
         second["rel_l2_vs_" + args.precision] = float(torch.linalg.vector_norm(du2 - du1) /
@@ -545,6 +569,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tflops(mixed), 2),
                      "unit": "TFLOP/s", "frac": round(achieved / peak_tflops(mixed), 4), "traffic": traffic,
                      "kernel": "k_rhs_tile x3 directions (one RHS)",
+                     "measured_roof": probe[roof_key],
+                     "frac_of_measured": round(achieved / probe[roof_key], 4), "fma_probe": probe,
                      "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
         "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
         "tile_shape/schedule": shape, "autotune_ms": tune,

@@ -56,7 +56,13 @@ extern "C" {
 #define WS_TILE_15x16_K1 0       /* x: 15x16, y/z: 16x15, one face per thread */
 #define WS_TILE_30x16_K2 1       /* x: 30x16, y/z: 32x15, two faces per thread (default) */
 #define WS_TILE_30x16_A16_K2 2   /* x: 30x16, y/z: 32x16 on 16-aligned columns, two faces per thread */
-#define WS_TILE_COUNT 3
+#define WS_TILE_62x8_K2 3        /* x: 62x8, y/z: 32x15, two faces per thread */
+#define WS_TILE_COUNT 4
+
+/* ws_probe_valu operations */
+#define WS_PROBE_FMA 0
+#define WS_PROBE_MUL 1
+#define WS_PROBE_ADD 2
 
 /* how the tiled kernels walk their tiles (ws_ctx_set_schedule, ws_batch_set_schedule) */
 #define WS_SCHED_GRID 0        /* one tile per work-group, grid = tile count */
@@ -231,6 +237,14 @@ int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipSt
  * precision.  times_ms (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major) may be NULL.  Overwrites st->du; synchronises the
  * stream.  No counterpart in the reference. */
 int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
+
+/* Vector ALU throughput of the current device in tera lane-operations per second: eight
+ * independent chains of one operation per lane over 16 work-groups of 256 lanes per
+ * compute unit, best of five timed launches.  precision WS_PREC_FP64 or WS_PREC_MIXED
+ * (scalar FP32); op one of WS_PROBE_*.  FMA counts one operation (two FLOP).  The
+ * attainable VALU roof beside the published peak.  Synchronises the stream.  No
+ * counterpart in the reference (its measurement asks for this check). */
+int ws_probe_valu(int32_t precision, int32_t op, double *tera_ops, hipStream_t stream);
 
 /* Viscous terms of the RHS added to du components 1..4 (the reference's du_vis_Jacobian3d,
  * src/solver/OCFD_Stream.cpp:229-276, with du_viscous_Jacobian3d_{init,[xyz]_init,[xyz]_final},

@@ -20,7 +20,9 @@ WS_SCHED_GRID, WS_SCHED_PERSISTENT, WS_SCHED_PREFETCH = 0, 1, 2
 SCHEDULES = (WS_SCHED_GRID, WS_SCHED_PERSISTENT, WS_SCHED_PREFETCH)
 WS_KERNEL_TILED, WS_KERNEL_ANCHOR = 0, 1
 WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2 = 0, 1, 2
-TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2)
+WS_TILE_62x8_K2 = 3
+WS_PROBE_FMA, WS_PROBE_MUL, WS_PROBE_ADD = 0, 1, 2
+TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2, WS_TILE_62x8_K2)
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32
@@ -106,6 +108,7 @@ _SIGS = {
                                  ctypes.c_void_p]),
     "ws_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.POINTER(ctypes.c_float),
                                    ctypes.c_void_p]),
+    "ws_probe_valu": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "ws_rk3_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_void_p]),
     "ws_rk3_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.c_double, _i32,
                                   ctypes.c_void_p]),

@@ -49,7 +49,7 @@ int fail(ws_batch *b, int r)
 void block_tiles(int b, const int n[3], int shape, int dir, std::vector<int4> &out)
 {
     const TileShapeDef &D = kTileShapes[shape];
-    const int L = dir == 0 ? D.L0 : D.L1, W = dir == 0 ? D.W0 : D.W1;
+    const int L = dir == 0 ? D.L0 : dir == 1 ? D.L1 : D.L2, W = dir == 0 ? D.W0 : dir == 1 ? D.W1 : D.W2;
     const int xa = tile_xalign(dir, W);
     // the box starts at LAP; aligned y/z tiles start at the aligned column below it
     const int nxa = n[0] + LAP - LAP / xa * xa;

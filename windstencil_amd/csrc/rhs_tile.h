@@ -40,10 +40,10 @@ struct Tile {
     // every tile stores its lines line-major (point p of line l at l * LS + p), so a
     // thread's window is contiguous and is read with 128-bit LDS loads.  x tiles fill
     // their lines along x (LS = NL); y/z tiles fill them across lines (consecutive
-    // threads write the same point of consecutive lines), so their line stride is
-    // padded to NL + 2 (= 2 mod 4 doubles): those writes spread over the banks and
-    // every line still starts 16-byte aligned
-    static constexpr int LS = DIR == 0 ? NL : NL + 2;
+    // threads write the same point of consecutive lines), so their line stride is the
+    // smallest LS >= NL with LS = 2 mod 4 doubles: those writes spread over the banks
+    // and every line still starts 16-byte aligned
+    static constexpr int LS = DIR == 0 ? NL : NL + (6 - NL % 4) % 4;
     static constexpr int QS = W * LS;                      // LDS entries per quantity
     static constexpr int NPTS = W * NL;
     static constexpr int ROUNDS = (NPTS + NT - 1) / NT;   // split points per thread (upper bound)
@@ -544,14 +544,14 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     }
 }
 
-template <int L0, int W0, int L1, int W1, int K, bool FIELDS, bool MIX>
+template <int L0, int W0, int L1, int W1, int L2, int W2, int K, bool FIELDS, bool MIX>
 void launch_all(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                 const ws_metric_fields &mf, ws_soa du, const Box &box, int accumulate, int sched, hipStream_t s)
 {
     if (accumulate) launch_dir<0, L0, W0, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
     else launch_dir<0, L0, W0, K, FIELDS, false, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
     launch_dir<1, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
-    launch_dir<2, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+    launch_dir<2, L2, W2, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
 }
 
 template <int S, bool FIELDS, bool MIX>
@@ -559,7 +559,7 @@ void launch_shape_n(const KParams &k, ws_field d, ws_field u, ws_field v, ws_fie
                     const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, hipStream_t s)
 {
     constexpr TileShapeDef D = kTileShapes[S];
-    launch_all<D.L0, D.W0, D.L1, D.W1, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s);
+    launch_all<D.L0, D.W0, D.L1, D.W1, D.L2, D.W2, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s);
 }
 
 template <bool FIELDS, bool MIX>
@@ -569,7 +569,8 @@ void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field 
     switch (shape) {
     case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
     case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
-    default: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
+    case 2: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
+    default: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
     }
 }
 
@@ -599,7 +600,7 @@ void launch_batch_shape_n(const KParams &k, const BatchTiles &bt, hipStream_t s)
     constexpr TileShapeDef D = kTileShapes[S];
     launch_batch_dir<0, D.L0, D.W0, D.K, false, MIX, PF>(k, bt, s);
     launch_batch_dir<1, D.L1, D.W1, D.K, true, MIX, PF>(k, bt, s);
-    launch_batch_dir<2, D.L1, D.W1, D.K, true, MIX, PF>(k, bt, s);
+    launch_batch_dir<2, D.L2, D.W2, D.K, true, MIX, PF>(k, bt, s);
 }
 
 template <bool MIX, bool PF>
@@ -608,7 +609,8 @@ void launch_batch_pf(const KParams &k, const BatchTiles &bt, hipStream_t s)
     switch (bt.shape) {
     case 0: launch_batch_shape_n<0, MIX, PF>(k, bt, s); break;
     case 1: launch_batch_shape_n<1, MIX, PF>(k, bt, s); break;
-    default: launch_batch_shape_n<2, MIX, PF>(k, bt, s); break;
+    case 2: launch_batch_shape_n<2, MIX, PF>(k, bt, s); break;
+    default: launch_batch_shape_n<3, MIX, PF>(k, bt, s); break;
     }
 }
 

@@ -120,67 +120,21 @@ __device__ __forceinline__ void split_ref(const KParams &k, double d, double u, 
 // Fast forms (tiled kernels)
 // ---------------------------------------------------------------------------
 
-// Nonlinear weights are homogeneous of degree four in (eps + S_k), so all
-// indicators are scaled by 1/720 (eps 2.592e-6/720 = 3.6e-9) and the
-// second/third-derivative coefficients become 780*36/720 = 39 and 781*36/720 = 39.05.
-constexpr double kEpsW = 3.6e-9;
-constexpr double kB = 39.0;
-constexpr double kC = 39.05;
+// Nonlinear weights are homogeneous of degree four in (eps + S_k), so the indicators may be
+// scaled freely.  The FP64 form scales them by 1/(720*39): the second-difference term
+// (780*36/720 = 39 in units of the 720 term) gets coefficient 1, the first-derivative
+// term 1/39 (folded into the candidates' difference coefficients as 1/sqrt(39)), the
+// third-difference term 39.05/39, and eps becomes 2.592e-6/(720*39).  The FP32 form keeps
+// the 1/720 scale (eps 3.6e-9): at 1/(720*39) the product of four (eps + S) factors of a
+// smooth window would fall below FP32's normal range.
 constexpr double kD0 = 0.0401954833730;
 constexpr double kD1 = 0.2493800006710;
 constexpr double kD2 = 0.4802686256260;
 constexpr double kD3 = 0.2009775476730;
 constexpr double kD4 = 0.0291783426580;
-
-// Positive reconstruction at face i-1/2 from s[0..7] = f(i-4..i+3) (REV = false), or the
-// negative one (REV = true, the positive rule on the reversed window).  Returns the
-// unnormalised pair (num, den) with value = num / (12*den).
-template <bool REV, typename Load>
-__device__ __forceinline__ void weno_fast(Load ld, double &num, double &den)
-{
-    double s[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) s[k] = REV ? ld(7 - k) : ld(k);
-    double F1[7], F2[6], F3[5];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) F1[k] = s[k + 1] - s[k];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) F2[k] = F1[k + 1] - F1[k];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) F3[k] = F2[k + 1] - F2[k];
-    // first-derivative terms a_k = 6 F1_k + beta_k F2_k + gamma_k F3_k (anchored at s_k)
-    const double a0 = fma(11.0, F3[0], fma(15.0, F2[0], 6.0 * F1[0]));
-    const double a1 = fma(2.0, F3[1], fma(9.0, F2[1], 6.0 * F1[1]));
-    const double a2 = fma(-1.0, F3[2], fma(3.0, F2[2], 6.0 * F1[2]));
-    const double a3 = fma(2.0, F3[3], fma(-3.0, F2[3], 6.0 * F1[3]));
-    const double a4 = fma(11.0, F3[4], fma(-9.0, F2[4], 6.0 * F1[4]));
-    // second-derivative terms (divided by 6)
-    const double b0 = fma(2.0, F3[0], F2[0]);
-    const double b1 = F2[2];
-    const double b2 = F2[3];
-    const double b3 = F2[3] - F3[3];
-    const double b4 = fma(-2.0, F3[4], F2[4]);
-    double S0 = fma(a0, a0, fma(kB * b0, b0, (kC * F3[0]) * F3[0]));
-    double S1 = fma(a1, a1, fma(kB * b1, b1, (kC * F3[1]) * F3[1]));
-    double S2 = fma(a2, a2, fma(kB * b2, b2, (kC * F3[2]) * F3[2]));
-    double S3 = fma(a3, a3, fma(kB * b3, b3, (kC * F3[3]) * F3[3]));
-    double S4 = fma(a4, a4, fma(kB * b4, b4, (kC * F3[4]) * F3[4]));
-    S4 = fmax(fmax(fmax(S0, S1), fmax(S2, S3)), S4);
-    const double e0 = kEpsW + S0, e1 = kEpsW + S1, e2 = kEpsW + S2, e3 = kEpsW + S3, e4 = kEpsW + S4;
-    const double p01 = e0 * e1, p012 = p01 * e2, s34 = e3 * e4, s234 = e2 * s34;
-    const double w0 = kD0 * (e1 * s234);
-    const double w1 = kD1 * (e0 * s234);
-    const double w2 = kD2 * (p01 * s34);
-    const double w3 = kD3 * (p012 * e4);
-    const double w4 = kD4 * (p012 * e3);
-    const double q0 = fma(25.0, s[3], fma(-23.0, s[2], fma(13.0, s[1], -3.0 * s[0])));
-    const double q1 = fma(3.0, s[4], fma(13.0, s[3], fma(-5.0, s[2], s[1])));
-    const double q2 = fma(7.0, s[3] + s[4], -(s[2] + s[5]));
-    const double q3 = fma(13.0, s[4], fma(-5.0, s[5], fma(3.0, s[3], s[6])));
-    const double q4 = fma(25.0, s[4], fma(-23.0, s[5], fma(13.0, s[6], -3.0 * s[7])));
-    num = fma(w4, q4, fma(w3, q3, fma(w2, q2, fma(w1, q1, w0 * q0))));
-    den = ((w0 + w1) + (w2 + w3)) + w4;
-}
+constexpr double kEps39 = 2.592e-6 / (720.0 * 39.0);
+constexpr double kC39 = 39.05 / 39.0;
+constexpr double kA39 = 0.16012815380508713;   // 1/sqrt(39)
 
 // Reciprocal: hardware estimate plus two Newton steps (error within a couple of ulp;
 // the argument is a positive normal number wherever it is used).
@@ -207,10 +161,10 @@ __device__ __forceinline__ double sqrt_nr(double x)
 }
 
 // K consecutive faces from K+7 points s[0..K+6] (face j uses s[j..j+7]).  The forward
-// differences, the six-fold first differences and the squared third (and shared
-// second) differences are formed once per point and shared by the K faces.  The
-// linear weights and the 1/12 of the candidates are folded into the candidate
-// coefficients, and the weighted sums are evaluated in Horner form over the
+// differences, the scaled first differences, the squared second differences and the
+// squared third differences (with eps added) are formed once per point and shared by the
+// K faces.  The linear weights and the 1/12 of the candidates are folded into the
+// candidate coefficients, and the weighted sums are evaluated in Horner form over the
 // prefix products of (eps + S_k).  Face j's value is num[j] / den[j].
 template <bool REV, int K, typename Load>
 __device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[K])
@@ -227,11 +181,11 @@ __device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[
     for (int i = 0; i < NP; ++i) s[i] = ld(i);
     double D1[NP - 1], D2[NP - 2], D3[NP - 3], G1[NP - 1], C3[NP - 3], B2[NP - 2];
 #pragma unroll
-    for (int i = 0; i < NP - 1; ++i) { D1[i] = s[i + 1] - s[i]; G1[i] = 6.0 * D1[i]; }
+    for (int i = 0; i < NP - 1; ++i) { D1[i] = s[i + 1] - s[i]; G1[i] = (6.0 * kA39) * D1[i]; }
 #pragma unroll
-    for (int i = 0; i < NP - 2; ++i) { D2[i] = D1[i + 1] - D1[i]; B2[i] = (kB * D2[i]) * D2[i]; }
+    for (int i = 0; i < NP - 2; ++i) { D2[i] = D1[i + 1] - D1[i]; B2[i] = D2[i] * D2[i]; }
 #pragma unroll
-    for (int i = 0; i < NP - 3; ++i) { D3[i] = D2[i + 1] - D2[i]; C3[i] = (kC * D3[i]) * D3[i]; }
+    for (int i = 0; i < NP - 3; ++i) { D3[i] = D2[i + 1] - D2[i]; C3[i] = fma(kC39 * D3[i], D3[i], kEps39); }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         // window point k of this face, and the anchors of candidate k's differences
@@ -239,26 +193,26 @@ __device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[
         auto i1 = [&](int k) { return REV ? j + 6 - k : j + k; };
         auto i2 = [&](int k) { return REV ? j + 5 - k : j + k; };
         auto i3 = [&](int k) { return REV ? j + 4 - k : j + k; };
-        const double sb = REV ? -1.0 : 1.0;
-        // first-derivative terms (sign of the whole term is irrelevant: it is squared)
-        const double a0 = fma(11.0, D3[i3(0)], fma(sb * 15.0, D2[i2(0)], G1[i1(0)]));
-        const double a1 = fma(2.0, D3[i3(1)], fma(sb * 9.0, D2[i2(1)], G1[i1(1)]));
-        const double a2 = fma(-1.0, D3[i3(2)], fma(sb * 3.0, D2[i2(2)], G1[i1(2)]));
-        const double a3 = fma(2.0, D3[i3(3)], fma(sb * -3.0, D2[i2(3)], G1[i1(3)]));
-        const double a4 = fma(11.0, D3[i3(4)], fma(sb * -9.0, D2[i2(4)], G1[i1(4)]));
+        const double sb = REV ? -kA39 : kA39;
+        // first-derivative terms over sqrt(39) (the sign of a term is irrelevant: it is squared)
+        const double a0 = fma(11.0 * kA39, D3[i3(0)], fma(sb * 15.0, D2[i2(0)], G1[i1(0)]));
+        const double a1 = fma(2.0 * kA39, D3[i3(1)], fma(sb * 9.0, D2[i2(1)], G1[i1(1)]));
+        const double a2 = fma(-1.0 * kA39, D3[i3(2)], fma(sb * 3.0, D2[i2(2)], G1[i1(2)]));
+        const double a3 = fma(2.0 * kA39, D3[i3(3)], fma(sb * -3.0, D2[i2(3)], G1[i1(3)]));
+        const double a4 = fma(11.0 * kA39, D3[i3(4)], fma(sb * -9.0, D2[i2(4)], G1[i1(4)]));
         // second-derivative terms of candidates 0, 3, 4 (1 and 2 are plain second differences)
         const double b0 = REV ? fma(-2.0, D3[j + 4], D2[j + 5]) : fma(2.0, D3[j], D2[j]);
         const double b3 = REV ? D2[j + 2] + D3[j + 1] : D2[j + 3] - D3[j + 3];
         const double b4 = REV ? fma(2.0, D3[j], D2[j + 1]) : fma(-2.0, D3[j + 4], D2[j + 4]);
         const double bb1 = REV ? B2[j + 3] : B2[j + 2];
         const double bb2 = REV ? B2[j + 2] : B2[j + 3];
-        const double S0 = fma(a0, a0, fma(kB * b0, b0, C3[i3(0)]));
-        const double S1 = fma(a1, a1, bb1 + C3[i3(1)]);
-        const double S2 = fma(a2, a2, bb2 + C3[i3(2)]);
-        const double S3 = fma(a3, a3, fma(kB * b3, b3, C3[i3(3)]));
-        double S4 = fma(a4, a4, fma(kB * b4, b4, C3[i3(4)]));
-        S4 = fmax(fmax(fmax(S0, S1), fmax(S2, S3)), S4);
-        const double e0 = kEpsW + S0, e1 = kEpsW + S1, e2 = kEpsW + S2, e3 = kEpsW + S3, e4 = kEpsW + S4;
+        // e_k = eps + S_k (eps rides in C3)
+        const double e0 = fma(a0, a0, fma(b0, b0, C3[i3(0)]));
+        const double e1 = fma(a1, a1, bb1 + C3[i3(1)]);
+        const double e2 = fma(a2, a2, bb2 + C3[i3(2)]);
+        const double e3 = fma(a3, a3, fma(b3, b3, C3[i3(3)]));
+        double e4 = fma(a4, a4, fma(b4, b4, C3[i3(4)]));
+        e4 = fmax(fmax(fmax(e0, e1), fmax(e2, e3)), e4);
         const double c0 = fma(q03, w(3), fma(q02, w(2), fma(q01, w(1), q00 * w(0))));
         const double c1 = fma(q14, w(4), fma(q13, w(3), fma(q12, w(2), q11 * w(1))));
         const double c2 = fma(q25, w(5), fma(q24, w(4), fma(q23, w(3), q22 * w(2))));
@@ -299,7 +253,7 @@ __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (
 #pragma unroll
     for (int i = 0; i < NP - 2; ++i) { D2[i] = D1[i + 1] - D1[i]; B2[i] = (fB * D2[i]) * D2[i]; }
 #pragma unroll
-    for (int i = 0; i < NP - 3; ++i) { D3[i] = D2[i + 1] - D2[i]; C3[i] = (fC * D3[i]) * D3[i]; }
+    for (int i = 0; i < NP - 3; ++i) { D3[i] = D2[i + 1] - D2[i]; C3[i] = fmaf(fC * D3[i], D3[i], fEps); }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         auto i1 = [&](int k) { return REV ? j + 6 - k : j + k; };
@@ -316,13 +270,13 @@ __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (
         const float b4 = REV ? fmaf(2.0f, D3[j], D2[j + 1]) : fmaf(-2.0f, D3[j + 4], D2[j + 4]);
         const float bb1 = REV ? B2[j + 3] : B2[j + 2];
         const float bb2 = REV ? B2[j + 2] : B2[j + 3];
-        const float S0 = fmaf(a0, a0, fmaf(fB * b0, b0, C3[i3(0)]));
-        const float S1 = fmaf(a1, a1, bb1 + C3[i3(1)]);
-        const float S2 = fmaf(a2, a2, bb2 + C3[i3(2)]);
-        const float S3 = fmaf(a3, a3, fmaf(fB * b3, b3, C3[i3(3)]));
-        float S4 = fmaf(a4, a4, fmaf(fB * b4, b4, C3[i3(4)]));
-        S4 = fmaxf(fmaxf(fmaxf(S0, S1), fmaxf(S2, S3)), S4);
-        const float e0 = fEps + S0, e1 = fEps + S1, e2 = fEps + S2, e3 = fEps + S3, e4 = fEps + S4;
+        // e_k = eps + S_k (eps rides in C3)
+        const float e0 = fmaf(a0, a0, fmaf(fB * b0, b0, C3[i3(0)]));
+        const float e1 = fmaf(a1, a1, bb1 + C3[i3(1)]);
+        const float e2 = fmaf(a2, a2, bb2 + C3[i3(2)]);
+        const float e3 = fmaf(a3, a3, fmaf(fB * b3, b3, C3[i3(3)]));
+        float e4 = fmaf(a4, a4, fmaf(fB * b4, b4, C3[i3(4)]));
+        e4 = fmaxf(fmaxf(fmaxf(e0, e1), fmaxf(e2, e3)), e4);
         // window relative to its base point w3: dw(k) = w(k) - w(3), from the FP32 differences
         // (P: w(k) = s[j+k], so w(k+1) - w(k) = D1[j+k]; M: w(k) = s[j+7-k], w(k+1) - w(k) = -D1[j+6-k])
         auto dw1 = [&](int k) { return REV ? -D1[j + 6 - k] : D1[j + k]; };  // w(k+1) - w(k)
