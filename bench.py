@@ -59,6 +59,8 @@ def parse():
                          "rk3: SSP-RK3 steps of one box in a graph")
     ap.add_argument("--rk3-steps", type=int, default=0, help="RK3 steps of the batched/rk3 workloads "
                                                             "(default: 10 batched, 100 rk3)")
+    ap.add_argument("--probe", action="store_true",
+                    help="only measure the device's VALU rates (ws_probe_valu) and print them")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for HBM counter calibration)")
     return ap.parse_args()
@@ -363,6 +365,10 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.probe:
+        L = W.lib()
+        print(json.dumps({"probe": "ws_probe_valu", **measured_peaks(L, W.stream_handle(torch.cuda.current_stream()))}))
+        return
     if args.workload == "batched":
         bench_batched(args)
         return
@@ -409,9 +415,10 @@ def main():
             times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
             W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 5, times, sh))
             tune = {f"{s}/{q}": round(times[s * ns + q], 4) for s in W.TILE_SHAPES for q in W.SCHEDULES}
-        chosen, sched = ctypes.c_int32(), ctypes.c_int32()
+        chosen, sched, order = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
         W.check(L.ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
+        W.check(L.ws_ctx_get_direction_order(ctx.handle, ctypes.byref(order)))
         for _ in range(args.warmup):
             rhs()
         torch.cuda.synchronize()
@@ -426,7 +433,7 @@ def main():
         torch.cuda.synchronize()
         prep = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
         kern = statistics.mean(e[1].elapsed_time(e[2]) for e in evs)
-        return wall / args.steps * 1e3, prep, kern, f"{chosen.value}/{sched.value}", tune
+        return wall / args.steps * 1e3, prep, kern, f"{chosen.value}/{sched.value}/{order.value}", tune
 
     other = W.WS_PREC_MIXED if prec == W.WS_PREC_FP64 else W.WS_PREC_FP64
     second = None
@@ -438,7 +445,7 @@ def main():
         peak2 = peak_tflops(other == W.WS_PREC_MIXED)
         second = {"precision": "mixed" if other == W.WS_PREC_MIXED else "fp64",
                   "value": round(N / (ms2 * 1e-3) / 1e6, 2), "ms_per_step": round(ms2, 4),
-                  "kernel_ms": round(kern2, 4), "tile_shape/schedule": shape2, "autotune_ms": tune2,
+                  "kernel_ms": round(kern2, 4), "tile_shape/schedule/order": shape2, "autotune_ms": tune2,
                   "roofline": {"bound": "valu", "achieved": round(ach2, 3), "peak": round(peak2, 2),
                                "unit": "TFLOP/s", "frac": round(ach2 / peak2, 4)}}
     ms, prep_ms, kern_ms, shape, tune = run(prec)
@@ -573,8 +580,9 @@ def main():
                      "frac_of_measured": round(achieved / probe[roof_key], 4), "fma_probe": probe,
                      "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
         "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
-        "tile_shape/schedule": shape, "autotune_ms": tune,
-        "autotune_key": "tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
+        "tile_shape/schedule/order": shape, "autotune_ms": tune,
+        "autotune_key": "tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch), "
+                        "each the faster direction order; order 0 = x, y, z, 1 = z, y, x",
         "parity_rel_l2_vs_anchor": parity,
         "other_precision": second,
         "mixed_vs_fp64_200_rk3_steps": steps200,

@@ -71,6 +71,12 @@ extern "C" {
                                   reconstruction */
 #define WS_SCHED_COUNT 3
 
+/* order of the three directional RHS launches (ws_ctx_set_direction_order): the first
+ * writes du, the other two accumulate; the orders agree to rounding */
+#define WS_ORDER_XYZ 0
+#define WS_ORDER_ZYX 1
+#define WS_ORDER_COUNT 2
+
 /* Ghosted scalar field; layout of cudaField under __HIPCC__ (cuda_commen.h:48-54). */
 typedef struct ws_field {
     double *ptr;
@@ -158,6 +164,11 @@ int ws_ctx_get_tile_shape(const ws_ctx *ctx, int32_t *shape);
  * grid schedule runs) */
 int ws_ctx_set_schedule(ws_ctx *ctx, int32_t schedule);
 int ws_ctx_get_schedule(const ws_ctx *ctx, int32_t *schedule);
+/* WS_ORDER_*: the launch that writes du reads nothing of it, the two after it
+ * read-modify-write du, and which direction pays for that read shifts the time per
+ * precision (the autotuner picks) */
+int ws_ctx_set_direction_order(ws_ctx *ctx, int32_t order);
+int ws_ctx_get_direction_order(const ws_ctx *ctx, int32_t *order);
 
 /* Allocate / free the solver state with the reference's extents
  * (replaces opencfd_mem_init_dev, src/runtime/OCFD_init.cpp:411-446). */
@@ -231,19 +242,21 @@ int ws_mem_value_init(double value, double *ptr, uint32_t pitch, uint32_t size_x
  * double).  No counterpart: the reference uses a fixed DT. */
 int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipStream_t stream);
 
-/* Time every (tile shape, schedule) pair of the tiled kernels, in the context's precision,
- * on this state (two untimed and `reps` timed RHS launches each, HIP events on `stream`),
- * keep the fastest for the context and remember it for contexts of the same extents and
- * precision.  times_ms (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major) may be NULL.  Overwrites st->du; synchronises the
+/* Time every (tile shape, schedule, direction order) of the tiled kernels, in the context's
+ * precision, on this state (two untimed and `reps` timed RHS launches each, HIP events on
+ * `stream`), keep the fastest for the context and remember it for contexts of the same
+ * extents and precision.  times_ms (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major,
+ * each the faster of the two orders) may be NULL.  Overwrites st->du; synchronises the
  * stream.  No counterpart in the reference. */
 int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
 
 /* Vector ALU throughput of the current device in tera lane-operations per second: eight
  * independent chains of one operation per lane over 16 work-groups of 256 lanes per
- * compute unit, best of five timed launches.  precision WS_PREC_FP64 or WS_PREC_MIXED
- * (scalar FP32); op one of WS_PROBE_*.  FMA counts one operation (two FLOP).  The
- * attainable VALU roof beside the published peak.  Synchronises the stream.  No
- * counterpart in the reference (its measurement asks for this check). */
+ * compute unit, best of five timed launches after 40 untimed ones.  precision
+ * WS_PREC_FP64 or WS_PREC_MIXED (scalar FP32); op one of WS_PROBE_*.  FMA counts one
+ * operation (two FLOP).  The attainable VALU roof beside the published peak.
+ * Synchronises the stream.  No counterpart in the reference (its measurement asks for
+ * this check). */
 int ws_probe_valu(int32_t precision, int32_t op, double *tera_ops, hipStream_t stream);
 
 /* Viscous terms of the RHS added to du components 1..4 (the reference's du_vis_Jacobian3d,

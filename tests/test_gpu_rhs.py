@@ -89,16 +89,53 @@ def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
     times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
     W.check(W.lib().ws_autotune(ctx.handle, ctypes.byref(st.ws), 3, times, W.stream_handle()))
     log(f"  autotune ms {list(times)}")
-    chosen, sched = ctypes.c_int32(), ctypes.c_int32()
+    chosen, sched, order = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
     W.check(W.lib().ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
+    W.check(W.lib().ws_ctx_get_direction_order(ctx.handle, ctypes.byref(order)))
     assert times[chosen.value * ns + sched.value] == min(times)
+    assert order.value in W.ORDERS
     ctx2 = make_ctx(nx, ny, nz)
-    again, again_s = ctypes.c_int32(), ctypes.c_int32()
+    again, again_s, again_o = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_ctx_get_tile_shape(ctx2.handle, ctypes.byref(again)))
     W.check(W.lib().ws_ctx_get_schedule(ctx2.handle, ctypes.byref(again_s)))
-    assert (again.value, again_s.value) == (chosen.value, sched.value)
+    W.check(W.lib().ws_ctx_get_direction_order(ctx2.handle, ctypes.byref(again_o)))
+    assert (again.value, again_s.value, again_o.value) == (chosen.value, sched.value, order.value)
     assert rel_l2(gpu_du_comput(ctx2, upload_state(prims)), ref) <= TOL_FP64
+
+
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+@pytest.mark.parametrize("tile", [W.WS_TILE_30x16_K2, W.WS_TILE_62x8_K2])
+def test_direction_orders_agree(orc, ws, tile, precision):
+    """Launching z first (it writes du, y and x accumulate) gives the x-first RHS to
+    rounding: FP64 within the drop-in bar of the oracle in both orders and 1e-13 of each
+    other, mixed within 1e-6 of the oracle; with accumulate = 1 both orders add the RHS
+    to what du held."""
+    import torch
+    nx, ny, nz = 67, 45, 37
+    prims = random_state(nx, ny, nz, 41, "shock")
+    ref = oracle_du(orc, prims)
+    tol = TOL_FP64 if precision == W.WS_PREC_FP64 else 1e-6
+    outs, accs = [], []
+    for order in W.ORDERS:
+        ctx = make_ctx(nx, ny, nz, precision=precision)
+        ctx.set_tile_shape(tile)
+        ctx.set_direction_order(order)
+        st = upload_state(prims)
+        du = gpu_du_comput(ctx, st)
+        err = rel_l2(du, ref)
+        log(f"  tile {tile} prec {precision} order {order}: rel L2 {err:.3e}")
+        assert err <= tol
+        outs.append(du)
+        # accumulate on top of a known du
+        st.interior("du").fill_(0.5)
+        torch.cuda.synchronize()
+        acc = gpu_du_comput(ctx, st, accumulate=True)
+        accs.append(acc)
+        assert rel_l2(acc - 0.5, ref) <= tol
+    if precision == W.WS_PREC_FP64:
+        assert rel_l2(outs[1], outs[0]) <= 1e-13
+        assert rel_l2(accs[1], accs[0]) <= 1e-13
 
 
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])

@@ -22,6 +22,8 @@ WS_KERNEL_TILED, WS_KERNEL_ANCHOR = 0, 1
 WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2 = 0, 1, 2
 WS_TILE_62x8_K2 = 3
 WS_PROBE_FMA, WS_PROBE_MUL, WS_PROBE_ADD = 0, 1, 2
+WS_ORDER_XYZ, WS_ORDER_ZYX = 0, 1
+ORDERS = (WS_ORDER_XYZ, WS_ORDER_ZYX)
 TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2, WS_TILE_62x8_K2)
 
 _u32 = ctypes.c_uint32
@@ -80,6 +82,8 @@ _SIGS = {
     "ws_ctx_get_tile_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_ctx_set_schedule": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_ctx_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_ctx_set_direction_order": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_ctx_get_direction_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_state_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_state_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p] + [WsField] * 5 +
@@ -240,6 +244,9 @@ class Context:
 
     def set_schedule(self, s):
         check(lib().ws_ctx_set_schedule(self.handle, s), "ws_ctx_set_schedule")
+
+    def set_direction_order(self, o):
+        check(lib().ws_ctx_set_direction_order(self.handle, o), "ws_ctx_set_direction_order")
 
 
 # ---- torch-backed buffers ---------------------------------------------------------------------

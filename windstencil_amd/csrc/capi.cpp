@@ -300,10 +300,11 @@ int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
     if (!c) return set_error(WS_ERR_NOMEM, "host allocation failed");
     c->cfg = *cfg;
     c->kp = make_kparams(*cfg);
-    int shape, sched;
-    if (remembered_choice(cfg->nx, cfg->ny, cfg->nz, cfg->precision, &shape, &sched)) {
+    int shape, sched, order;
+    if (remembered_choice(cfg->nx, cfg->ny, cfg->nz, cfg->precision, &shape, &sched, &order)) {
         c->tile_shape = shape;
         c->schedule = sched;
+        c->dir_order = order;
     }
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return hip_error(e, "hipGetDevice"); }
@@ -341,10 +342,11 @@ int ws_ctx_set_precision(ws_ctx *c, int32_t precision)
     if (!c) return set_error(WS_ERR_INVALID, "null context");
     if (precision != WS_PREC_FP64 && precision != WS_PREC_MIXED) return set_error(WS_ERR_INVALID, "unknown precision");
     c->cfg.precision = precision;
-    int shape, sched;
-    if (remembered_choice(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision, &shape, &sched)) {
+    int shape, sched, order;
+    if (remembered_choice(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision, &shape, &sched, &order)) {
         c->tile_shape = shape;
         c->schedule = sched;
+        c->dir_order = order;
     }
     return WS_OK;
 }
@@ -377,6 +379,21 @@ int ws_ctx_get_schedule(const ws_ctx *c, int32_t *schedule)
 {
     if (!c || !schedule) return set_error(WS_ERR_INVALID, "null argument");
     *schedule = c->schedule;
+    return WS_OK;
+}
+
+int ws_ctx_set_direction_order(ws_ctx *c, int32_t order)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (order < 0 || order >= WS_ORDER_COUNT) return set_error(WS_ERR_INVALID, "unknown direction order");
+    c->dir_order = order;
+    return WS_OK;
+}
+
+int ws_ctx_get_direction_order(const ws_ctx *c, int32_t *order)
+{
+    if (!c || !order) return set_error(WS_ERR_INVALID, "null argument");
+    *order = c->dir_order;
     return WS_OK;
 }
 

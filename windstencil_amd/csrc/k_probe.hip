@@ -63,8 +63,10 @@ int probe(double *rate, hipStream_t s)
     } else {
         const T a = (T)0.999999, b = (T)1e-6;
         float best = 1e30f;
-        // one untimed launch, then the best of five
-        for (int rep = 0; rep < 6 && !r; ++rep) {
+        // WARM untimed launches (about 40 ms: the clock settles at its loaded value), then
+        // the best of five
+        constexpr int WARM = 40;
+        for (int rep = 0; rep < WARM + 5 && !r; ++rep) {
             if ((e = hipEventRecord(e0, s)) != hipSuccess) { r = hip_error(e, "probe event"); break; }
             hipLaunchKernelGGL((k_valu_probe<T, OP>), dim3(blocks), dim3(PT), 0, s, a, b, iters, out);
             if ((e = hipGetLastError()) != hipSuccess) { r = hip_error(e, "probe launch"); break; }
@@ -74,7 +76,7 @@ int probe(double *rate, hipStream_t s)
             }
             float ms = 0.f;
             hipEventElapsedTime(&ms, e0, e1);
-            if (rep > 0 && ms < best) best = ms;
+            if (rep >= WARM && ms < best) best = ms;
         }
         if (!r) {
             const double ops = (double)CHAINS * UNROLL * (double)iters * (double)blocks * PT;

@@ -544,10 +544,21 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     }
 }
 
+// The first launch writes du (or accumulates into it when the caller asks), the other two
+// read-modify-write it.  Which direction pays for that read shifts the time differently
+// per precision, so the order is a tuning choice (WS_ORDER_*).
 template <int L0, int W0, int L1, int W1, int L2, int W2, int K, bool FIELDS, bool MIX>
 void launch_all(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                const ws_metric_fields &mf, ws_soa du, const Box &box, int accumulate, int sched, hipStream_t s)
+                const ws_metric_fields &mf, ws_soa du, const Box &box, int accumulate, int sched, int order,
+                hipStream_t s)
 {
+    if (order == WS_ORDER_ZYX) {
+        if (accumulate) launch_dir<2, L2, W2, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+        else launch_dir<2, L2, W2, K, FIELDS, false, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+        launch_dir<1, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+        launch_dir<0, L0, W0, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+        return;
+    }
     if (accumulate) launch_dir<0, L0, W0, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
     else launch_dir<0, L0, W0, K, FIELDS, false, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
     launch_dir<1, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
@@ -556,21 +567,24 @@ void launch_all(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
 
 template <int S, bool FIELDS, bool MIX>
 void launch_shape_n(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                    const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, hipStream_t s)
+                    const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, int order,
+                    hipStream_t s)
 {
     constexpr TileShapeDef D = kTileShapes[S];
-    launch_all<D.L0, D.W0, D.L1, D.W1, D.L2, D.W2, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s);
+    launch_all<D.L0, D.W0, D.L1, D.W1, D.L2, D.W2, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched,
+                                                                    order, s);
 }
 
 template <bool FIELDS, bool MIX>
 void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                  const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, hipStream_t s)
+                  const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, int order,
+                  hipStream_t s)
 {
     switch (shape) {
-    case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
-    case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
-    case 2: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
-    default: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, s); break;
+    case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
+    case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
+    case 2: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
+    default: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
     }
 }
 
@@ -625,9 +639,9 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
     const int shape = c.tile_shape;
     if (m) {
         mf = *m;
-        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, s);
+        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order, s);
     } else {
-        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, s);
+        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order, s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, MIX ? "mixed-precision RHS launch" : "tiled RHS launch");

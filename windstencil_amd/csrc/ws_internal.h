@@ -52,6 +52,8 @@ struct Ctx {
     int tile_shape = WS_TILE_30x16_K2;
     // how the tiled kernels walk their tiles (WS_SCHED_*)
     int schedule = WS_SCHED_GRID;
+    // order of the directional launches (WS_ORDER_*)
+    int dir_order = WS_ORDER_XYZ;
     // viscous terms: twelve ghosted flux fields, allocated at the first use
     double *vis_scratch = nullptr;
     size_t vis_scratch_elems = 0;
@@ -78,9 +80,9 @@ struct BatchTiles {
     int ntiles[3];
 };
 
-// (tile shape, schedule) the autotuner found best for these extents and precision;
-// false if none
-bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule);
+// (tile shape, schedule, direction order) the autotuner found best for these extents and
+// precision; false if none
+bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order);
 
 // error reporting (thread-local message)
 int set_error(int code, const std::string &msg);
