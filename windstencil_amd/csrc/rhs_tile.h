@@ -167,6 +167,31 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     const double cSn = k.Anorm[DIR];
     const double cK1 = k.Ak[3 * DIR + 0], cK2 = k.Ak[3 * DIR + 1], cK3 = k.Ak[3 * DIR + 2];
 
+    // ---- phase 3 prologue: accumulating launches read du; those loads go out with the
+    // tile's primitive loads, so one wait at the split covers both (issued later, the
+    // wait the compiler places before the reconstruction exposes their whole latency)
+    constexpr int CPT = (W * L + TL::NT - 1) / TL::NT;    // cells per thread (upper bound)
+    const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
+    double duo[ACC ? CPT : 1][5];
+    if (ACC) {
+#pragma unroll
+        for (int r = 0; r < CPT; ++r) {
+            const int c = t + r * TL::NT;
+            int l, i;
+            if (DIR == 0) { i = c % L; l = c / L; }
+            else          { l = c % W; i = c / W; }
+            if (c >= W * L || l < g.lo_across || l >= g.lim_across || i >= g.lim_line) continue;
+            int cx, cy, cz;
+            if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
+            else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
+            else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
+            const double *dp = du.ptr + (size_t)(cx - LAP) +
+                               (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
+#pragma unroll
+            for (int m = 0; m < 5; ++m) duo[r][m] = dp[m * vstride];
+        }
+    }
+
     // ---- phase 1b: split every point of the tile into LDS --------------------------------
 #pragma unroll
     for (int r = 0; r < TL::ROUNDS; ++r) {
@@ -198,30 +223,6 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     }
     next();
     __syncthreads();
-
-    // ---- phase 3 prologue: accumulating launches read du; issue those loads now so
-    // they complete while this thread reconstructs its faces
-    constexpr int CPT = (W * L + TL::NT - 1) / TL::NT;    // cells per thread (upper bound)
-    const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
-    double duo[ACC ? CPT : 1][5];
-    if (ACC) {
-#pragma unroll
-        for (int r = 0; r < CPT; ++r) {
-            const int c = t + r * TL::NT;
-            int l, i;
-            if (DIR == 0) { i = c % L; l = c / L; }
-            else          { l = c % W; i = c / W; }
-            if (c >= W * L || l < g.lo_across || l >= g.lim_across || i >= g.lim_line) continue;
-            int cx, cy, cz;
-            if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
-            else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
-            else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
-            const double *dp = du.ptr + (size_t)(cx - LAP) +
-                               (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
-#pragma unroll
-            for (int m = 0; m < 5; ++m) duo[r][m] = dp[m * vstride];
-        }
-    }
 
     // ---- phase 2: K faces of one line per thread, all five variables --------------------
     const int line = t / TL::T, seg = t % TL::T;
