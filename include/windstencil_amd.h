@@ -146,6 +146,19 @@ typedef struct ws_state {
     ws_field d, u, v, w, T, P, cc;          /* pd_d .. pP_d, pcc_d */
 } ws_state;
 
+/* State analysis (ws_ana_state).  First offending cells in interior coordinates (ghost
+ * cells are negative or >= n); WS_ANA_NONE when there is none. */
+#define WS_ANA_NONE (-2147483647 - 1)
+typedef struct ws_ana_report {
+    int64_t nan_T;              /* ghosted cells whose T is NaN */
+    int64_t neg_T;              /* ghosted cells with T < 0 */
+    int64_t neg_d;              /* ghosted cells with density < 0 (no counterpart) */
+    int64_t neg_jac;            /* ghosted cells with Ajac < 0 */
+    int32_t first_nan_T[3], first_neg_T[3], first_neg_d[3], first_neg_jac[3];
+    double cfl;                 /* dt * max(|u|,|v|,|w|) + c / min(h) over the interior (0 if dt <= 0) */
+    int32_t fatal;              /* NaN T or more than 10 cells with T < 0: where the reference aborts */
+} ws_ana_report;
+
 /* ---- library ------------------------------------------------------------ */
 const char *ws_version(void);
 const char *ws_error_string(int code);
@@ -258,6 +271,16 @@ int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, 
  * Synchronises the stream.  No counterpart in the reference (its measurement asks for
  * this check). */
 int ws_probe_valu(int32_t precision, int32_t op, double *tera_ops, hipStream_t stream);
+
+/* Failure detection on the device, one pass over the ghosted box: cells whose T is NaN or
+ * negative (the reference's ana_NAN_and_NT, src/runtime/OCFD_ana.cpp:115-225, which scans
+ * T on the host and aborts on a NaN or on more than 10 negative cells: report->fatal), cells
+ * with Ajac < 0 (ana_Jac, :35-56; metrics NULL checks the context's constant Ajac), cells
+ * with negative density, and the CFL number of the time step dt (dt <= 0: skipped).  Counts
+ * and first cells go to *report; the call returns WS_OK whatever it finds and synchronises
+ * the stream. */
+int ws_ana_state(ws_ctx *ctx, const ws_state *st, const ws_metric_fields *metrics, double dt,
+                 ws_ana_report *report, hipStream_t stream);
 
 /* Viscous terms of the RHS added to du components 1..4 (the reference's du_vis_Jacobian3d,
  * src/solver/OCFD_Stream.cpp:229-276, with du_viscous_Jacobian3d_{init,[xyz]_init,[xyz]_final},

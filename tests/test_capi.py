@@ -53,6 +53,9 @@ def test_descriptor_layouts_match_the_reference(ws):
     assert ctypes.sizeof(ws.WsField) == 16
     assert ctypes.sizeof(ws.WsSoa) == 24
     assert ctypes.sizeof(ws.WsJob) == 24
+    # ws_ana_report: four int64 counts, four int32[3] cells, the CFL number, the flag
+    assert ctypes.sizeof(ws.WsAnaReport) == 96
+    assert ws.WsAnaReport.cfl.offset == 80 and ws.WsAnaReport.fatal.offset == 88
 
 
 def test_host_side_validation_without_gpu(ws):
@@ -72,6 +75,10 @@ def test_host_side_validation_without_gpu(ws):
     bad = ws.default_config(64, 32, 16, gamma=1.0)
     assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -1
     assert L.ws_error_string(-3) == b"HIP runtime error"
+    rep = ws.WsAnaReport()
+    assert L.ws_ana_state(None, None, None, 0.0, ctypes.byref(rep), None) == -1
+    v = ctypes.c_double()
+    assert L.ws_probe_valu(ws.WS_PREC_FP64, ws.WS_PROBE_FMA, None, None) == -1
 
 
 def test_product_path_has_no_oracle_dependency():

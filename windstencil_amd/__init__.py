@@ -67,6 +67,16 @@ class WsState(ctypes.Structure):
                 ("T", WsField), ("P", WsField), ("cc", WsField)]
 
 
+class WsAnaReport(ctypes.Structure):
+    _fields_ = [("nan_T", ctypes.c_int64), ("neg_T", ctypes.c_int64), ("neg_d", ctypes.c_int64),
+                ("neg_jac", ctypes.c_int64), ("first_nan_T", _i32 * 3), ("first_neg_T", _i32 * 3),
+                ("first_neg_d", _i32 * 3), ("first_neg_jac", _i32 * 3), ("cfl", ctypes.c_double),
+                ("fatal", _i32)]
+
+
+WS_ANA_NONE = -2147483648
+
+
 # every extern "C" symbol of include/*.h, with its ctypes signature
 _SIGS = {
     "ws_version": (ctypes.c_char_p, []),
@@ -113,6 +123,8 @@ _SIGS = {
     "ws_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.POINTER(ctypes.c_float),
                                    ctypes.c_void_p]),
     "ws_probe_valu": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "ws_ana_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_double,
+                                    ctypes.POINTER(WsAnaReport), ctypes.c_void_p]),
     "ws_rk3_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_void_p]),
     "ws_rk3_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.c_double, _i32,
                                   ctypes.c_void_p]),
