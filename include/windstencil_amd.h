@@ -57,7 +57,8 @@ extern "C" {
 #define WS_TILE_30x16_K2 1       /* x: 30x16, y/z: 32x15, two faces per thread (default) */
 #define WS_TILE_30x16_A16_K2 2   /* x: 30x16, y/z: 32x16 on 16-aligned columns, two faces per thread */
 #define WS_TILE_62x8_K2 3        /* x: 62x8, y/z: 32x15, two faces per thread */
-#define WS_TILE_COUNT 4
+#define WS_TILE_126x2_K2 4       /* x: 126x2 (two-wavefront work-groups), y/z: 32x15 */
+#define WS_TILE_COUNT 5
 
 /* ws_probe_valu operations */
 #define WS_PROBE_FMA 0

@@ -21,10 +21,11 @@ SCHEDULES = (WS_SCHED_GRID, WS_SCHED_PERSISTENT, WS_SCHED_PREFETCH)
 WS_KERNEL_TILED, WS_KERNEL_ANCHOR = 0, 1
 WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2 = 0, 1, 2
 WS_TILE_62x8_K2 = 3
+WS_TILE_126x2_K2 = 4
 WS_PROBE_FMA, WS_PROBE_MUL, WS_PROBE_ADD = 0, 1, 2
 WS_ORDER_XYZ, WS_ORDER_ZYX = 0, 1
 ORDERS = (WS_ORDER_XYZ, WS_ORDER_ZYX)
-TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2, WS_TILE_62x8_K2)
+TILE_SHAPES = (WS_TILE_15x16_K1, WS_TILE_30x16_K2, WS_TILE_30x16_A16_K2, WS_TILE_62x8_K2, WS_TILE_126x2_K2)
 
 _u32 = ctypes.c_uint32
 _i32 = ctypes.c_int32

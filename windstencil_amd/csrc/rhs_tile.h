@@ -585,7 +585,8 @@ void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field 
     case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
     case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
     case 2: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
-    default: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
+    case 3: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
+    default: launch_shape_n<4, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
     }
 }
 
@@ -625,7 +626,8 @@ void launch_batch_pf(const KParams &k, const BatchTiles &bt, hipStream_t s)
     case 0: launch_batch_shape_n<0, MIX, PF>(k, bt, s); break;
     case 1: launch_batch_shape_n<1, MIX, PF>(k, bt, s); break;
     case 2: launch_batch_shape_n<2, MIX, PF>(k, bt, s); break;
-    default: launch_batch_shape_n<3, MIX, PF>(k, bt, s); break;
+    case 3: launch_batch_shape_n<3, MIX, PF>(k, bt, s); break;
+    default: launch_batch_shape_n<4, MIX, PF>(k, bt, s); break;
     }
 }
 
