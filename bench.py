@@ -59,6 +59,9 @@ def parse():
                          "rk3: SSP-RK3 steps of one box in a graph")
     ap.add_argument("--rk3-steps", type=int, default=0, help="RK3 steps of the batched/rk3 workloads "
                                                             "(default: 10 batched, 100 rk3)")
+    ap.add_argument("--stage-fusion", type=int, default=1, choices=[0, 1],
+                    help="rk3 workload: 1 lets the last RHS launch apply each stage (default), 0 the "
+                         "separate stage kernel")
     ap.add_argument("--probe", action="store_true",
                     help="only measure the device's VALU rates (ws_probe_valu) and print them")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -318,6 +321,7 @@ def bench_rk3(args):
     stream = torch.cuda.current_stream()
     sh = W.stream_handle(stream)
     ctx = W.Context(W.default_config(nx, ny, nz, precision=prec))
+    ctx.set_stage_fusion(args.stage_fusion)
     st = W.TorchState(nx, ny, nz)
     st.set_prims(*prims)
     W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
@@ -350,6 +354,7 @@ def bench_rk3(args):
                                "one hipGraph per step", "grid": [nx, ny, nz], "precision": args.precision,
                    "cells": N},
                   roofline=roof, rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
+                  stage_fusion=args.stage_fusion,
                   totals=[float(v) for v in f.sum(dim=(1, 2, 3)).cpu()])
     out["steps"] = nsteps
     print(json.dumps(out))

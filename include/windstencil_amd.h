@@ -183,6 +183,10 @@ int ws_ctx_get_schedule(const ws_ctx *ctx, int32_t *schedule);
  * precision (the autotuner picks) */
 int ws_ctx_set_direction_order(ws_ctx *ctx, int32_t order);
 int ws_ctx_get_direction_order(const ws_ctx *ctx, int32_t *order);
+/* 1 (default): RK3 steps let the last directional RHS launch apply each stage's update
+ * (tiled kernels, grid schedule, constant metrics, no closures, no viscous terms; same
+ * result as the separate stage kernel, du then holds partial sums); 0: separate update. */
+int ws_ctx_set_stage_fusion(ws_ctx *ctx, int32_t on);
 
 /* Allocate / free the solver state with the reference's extents
  * (replaces opencfd_mem_init_dev, src/runtime/OCFD_init.cpp:411-446). */
@@ -298,7 +302,9 @@ int ws_viscous_rhs(ws_ctx *ctx, ws_field u, ws_field v, ws_field w, ws_field T, 
 
 /* One SSP-RK3 step (the reference loop body, src/solver/OCFD_NS_Solver.cpp:64-90, with
  * all three stages): swap f/fn, then per stage du_comput, time advance, get_duvwT and
- * (if enabled) OCFD_bc.  dt_dev: device double holding dt, or NULL to use cfg.dt. */
+ * (if enabled) OCFD_bc.  dt_dev: device double holding dt, or NULL to use cfg.dt.  du is
+ * scratch here: with stage fusion (ws_ctx_set_stage_fusion) it does not end up holding
+ * the last stage's RHS. */
 int ws_rk3_step(ws_ctx *ctx, ws_state *st, const double *dt_dev, hipStream_t stream);
 
 /* nsteps RK3 steps.  cfl > 0: dt recomputed on the device at the start of every step

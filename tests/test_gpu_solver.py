@@ -123,11 +123,15 @@ def test_time_advance_stages_bitwise(orc, ws):
         assert np.array_equal(st.interior("f").cpu().numpy(), fc), krk
 
 
-def _gpu_run(prims, nsteps, kernel, graph, cfl=0.5):
+def _gpu_run(prims, nsteps, kernel, graph, cfl=0.5, precision=W.WS_PREC_FP64, order=None, fusion=None):
     nz, ny, nx = prims[0].shape
     L = W.lib()
     s = W.stream_handle()
-    ctx = make_ctx(nx, ny, nz, kernel=kernel)
+    ctx = make_ctx(nx, ny, nz, kernel=kernel, precision=precision)
+    if order is not None:
+        ctx.set_direction_order(order)
+    if fusion is not None:
+        ctx.set_stage_fusion(fusion)
     st = upload_state(prims)
     W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
                              W.job((0, 0, 0), (nx, ny, nz)), s))
@@ -166,6 +170,22 @@ def test_rk3_tiled_matches_oracle(orc, ws, kind):
     err = rel_l2(f_gpu, f_ref)
     log(f"  rel L2 {err:.3e}")
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+@pytest.mark.parametrize("order", [W.WS_ORDER_XYZ, W.WS_ORDER_ZYX])
+@pytest.mark.parametrize("graph", [False, True])
+def test_stage_applied_by_the_last_rhs_launch_equals_the_stage_kernel(ws, precision, order, graph):
+    """RK3 steps whose last directional RHS launch applies the stage (the default) against
+    the separate stage kernel reading du back: the same stage expression on the same
+    complete RHS, so the states agree bit for bit (boundary planes on, CFL step from the
+    device)."""
+    prims = random_state(45, 26, 19, 13, "shock")
+    a, _ = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=True)
+    b, _ = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=False)
+    log(f"  fused vs separate stage: prec {precision} order {order} graph {graph}: max |diff| "
+        f"{np.abs(a - b).max():.3e}, rel L2 {rel_l2(a, b):.3e}")
+    assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("nsteps", [2, 3, 4, 5])

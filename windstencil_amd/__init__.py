@@ -95,6 +95,7 @@ _SIGS = {
     "ws_ctx_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_ctx_set_direction_order": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_ctx_get_direction_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_ctx_set_stage_fusion": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_state_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_state_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p] + [WsField] * 5 +
@@ -260,6 +261,9 @@ class Context:
 
     def set_direction_order(self, o):
         check(lib().ws_ctx_set_direction_order(self.handle, o), "ws_ctx_set_direction_order")
+
+    def set_stage_fusion(self, on):
+        check(lib().ws_ctx_set_stage_fusion(self.handle, 1 if on else 0), "ws_ctx_set_stage_fusion")
 
 
 # ---- torch-backed buffers ---------------------------------------------------------------------

@@ -230,18 +230,37 @@ int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s)
 // copy interior cells, whose sound speed is computed by the same expression).  With a
 // non-periodic direction the caller's ghost layers of T feed the sound speed there, so
 // every stage then prepares its inputs the separate way.
+//
+// With the tiled kernels, the grid schedule, constant metrics, no closures and no viscous
+// terms, the last directional RHS launch applies the stage to its cells itself (the stage
+// value from fn, f and the complete RHS in registers, the same expression as the stage
+// kernel) and the stage kernel then only derives the primitives: du never carries the
+// complete RHS, which saves its write and read-back.
 int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
 {
     std::swap(st->f, st->fn);
     const ws_soa nolap{};
     const bool periodic = c->cfg.periodic[0] && c->cfg.periodic[1] && c->cfg.periodic[2];
+    const bool closures = c->kp.bnd_any[0] || c->kp.bnd_any[1] || c->kp.bnd_any[2];
+    const bool fuse = c->fuse_stage && c->cfg.kernel == WS_KERNEL_TILED && !c->cfg.viscous &&
+                      c->schedule == WS_SCHED_GRID && !closures;
     const ws_field g[6] = {st->d, st->u, st->v, st->w, st->T, st->cc};
     int r = WS_OK;
     for (int krk = 1; krk <= 3 && !r; ++krk) {
         if (krk == 1 || !periodic) r = prepare_rhs_inputs(c, st, s);
-        if (!r) r = du_from_prepared(c, st, s);
         const bool sound = periodic && krk < 3;
-        if (!r) r = launch_stage(*c, krk, *st, nolap, c->cfg.dt, dt_dev, c->cfg.bc_planes != 0, sound, s);
+        const bool planes = c->cfg.bc_planes != 0;
+        if (fuse) {
+            const StageArgs sa{st->f, st->fn, dt_dev, c->cfg.dt, krk};
+            if (!r)
+                r = c->cfg.precision == WS_PREC_MIXED
+                        ? launch_rhs_mixed_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, interior_job(c), sa, s)
+                        : launch_rhs_tiled_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, interior_job(c), sa, s);
+            if (!r) r = launch_stage(*c, 0, *st, nolap, c->cfg.dt, dt_dev, planes, sound, s);
+        } else {
+            if (!r) r = du_from_prepared(c, st, s);
+            if (!r) r = launch_stage(*c, krk, *st, nolap, c->cfg.dt, dt_dev, planes, sound, s);
+        }
         if (!r && sound) r = launch_ghost_fill(*c, g, 6, s);
     }
     return r;
@@ -394,6 +413,13 @@ int ws_ctx_get_direction_order(const ws_ctx *c, int32_t *order)
 {
     if (!c || !order) return set_error(WS_ERR_INVALID, "null argument");
     *order = c->dir_order;
+    return WS_OK;
+}
+
+int ws_ctx_set_stage_fusion(ws_ctx *c, int32_t on)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    c->fuse_stage = on ? 1 : 0;
     return WS_OK;
 }
 

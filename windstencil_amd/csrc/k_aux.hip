@@ -149,16 +149,7 @@ __global__ void __launch_bounds__(NT) k_time_advance(KParams k, ws_soa f, ws_soa
         const size_t a = sidx(f.pitch, k.ny, k.nz, x, y, z, m);
         const double fnv = fn.ptr[sidx(fn.pitch, k.ny, k.nz, x, y, z, m)];
         const double duv = du.ptr[sidx(du.pitch, k.ny, k.nz, x, y, z, m)];
-        double r;
-        if (KRK == 1) {
-            r = fnv + dt * duv;
-        } else if (KRK == 2) {
-            const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
-            r = t1 * fnv + t2 * (f.ptr[a] + dt * duv);
-        } else {
-            const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
-            r = t1 * fnv + t2 * (f.ptr[a] + dt * duv);
-        }
+        const double r = rk3_stage_value<KRK>(fnv, KRK == 1 ? 0.0 : f.ptr[a], duv, dt);
         f.ptr[a] = r;
         if (KRK == 3 && fl.ptr)
             fl.ptr[(size_t)(x + LAP) + (size_t)fl.pitch * ((size_t)(y + LAP) + (size_t)(k.ny + 2 * LAP) *
@@ -171,7 +162,9 @@ __global__ void __launch_bounds__(NT) k_time_advance(KParams k, ws_soa f, ws_soa
 // boundary planes from those primitives (as k_pri_to_cons over the planes), and with
 // SOUND the sound speed of the interior (as k_sound_speed; a ghost fill then copies it).
 // Same expressions with contraction off as the separate kernels, so the state is
-// bit-identical; f is read once and the primitives are not read back.
+// bit-identical; f is read once and the primitives are not read back.  KRK = 0: the last
+// RHS launch already wrote the stage's f; this reads it, writes the primitives, and
+// writes f back only on the boundary planes it re-derives.
 template <int KRK, bool PLANES, bool SOUND>
 __global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl, double dt_val,
                                               const double *dt_dev, ws_field d, ws_field u, ws_field v, ws_field w,
@@ -181,24 +174,21 @@ __global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws
     const long total = (long)k.nx * k.ny * k.nz;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const double dt = dt_dev ? *dt_dev : dt_val;
+    const double dt = (KRK != 0 && dt_dev) ? *dt_dev : dt_val;
     const int x = (int)(i % k.nx);
     const int y = (int)((i / k.nx) % k.ny);
     const int z = (int)(i / ((long)k.nx * k.ny));
     double r[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
+        if (KRK == 0) {
+            r[m] = f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)];
+            continue;
+        }
         const double fnv = fn.ptr[sidx(fn.pitch, k.ny, k.nz, x, y, z, m)];
         const double duv = du.ptr[sidx(du.pitch, k.ny, k.nz, x, y, z, m)];
-        if (KRK == 1) {
-            r[m] = fnv + dt * duv;
-        } else if (KRK == 2) {
-            const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
-            r[m] = t1 * fnv + t2 * (f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] + dt * duv);
-        } else {
-            const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
-            r[m] = t1 * fnv + t2 * (f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] + dt * duv);
-        }
+        r[m] = rk3_stage_value<KRK == 0 ? 1 : KRK>(fnv, KRK == 1 ? 0.0 : f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)],
+                                                   duv, dt);
         if (KRK == 3 && fl.ptr)
             fl.ptr[(size_t)(x + LAP) + (size_t)fl.pitch * ((size_t)(y + LAP) + (size_t)(k.ny + 2 * LAP) *
                    ((size_t)(z + LAP) + (size_t)(k.nz + 2 * LAP) * m))] = r[m];
@@ -216,15 +206,17 @@ __global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws
     T.ptr[fidx(T.pitch, k.nyg, gx, gy, gz)] = TT;
     if (P.ptr) P.ptr[fidx(P.pitch, k.nyg, gx, gy, gz)] = T1 / d1 * (k.gamma - 1.0);
     if (SOUND) cc.ptr[fidx(cc.pitch, k.nyg, gx, gy, gz)] = sqrt(TT) / k.ama;
-    if (PLANES && (x == 0 || x == k.nx - 1 || y == 0 || y == k.ny - 1 || z == 0 || z == k.nz - 1)) {
+    const bool plane = PLANES && (x == 0 || x == k.nx - 1 || y == 0 || y == k.ny - 1 || z == 0 || z == k.nz - 1);
+    if (plane) {
         r[0] = d1;
         r[1] = u1 * d1;
         r[2] = v1 * d1;
         r[3] = w1 * d1;
         r[4] = d1 * ((u1 * u1 + v1 * v1 + w1 * w1) * 0.5 + k.cv * TT);
     }
+    if (KRK != 0 || plane)
 #pragma unroll
-    for (int m = 0; m < 5; ++m) f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] = r[m];
+        for (int m = 0; m < 5; ++m) f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] = r[m];
 }
 
 __global__ void __launch_bounds__(NT) k_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy,
@@ -386,8 +378,14 @@ int launch_stage(const Ctx &c, int krk, const ws_state &st, ws_soa fl, double dt
         hipLaunchKernelGGL(kern, g, dim3(NT), 0, s, k, st.f, st.fn, st.du, fl, dt, dt_dev, st.d, st.u, st.v, st.w,
                            st.T, st.P, st.cc);
     };
-    const int sel = (krk - 1) * 4 + (planes ? 2 : 0) + (sound ? 1 : 0);
+    // krk 0: the primitives (and planes, sound speed) of an f the last RHS launch updated
+    if (krk < 0 || krk > 3) return set_error(WS_ERR_INVALID, "krk must be 0, 1, 2 or 3");
+    const int sel = (krk == 0 ? 12 : (krk - 1) * 4) + (planes ? 2 : 0) + (sound ? 1 : 0);
     switch (sel) {
+    case 12: go(k_stage<0, false, false>); break;
+    case 13: go(k_stage<0, false, true>); break;
+    case 14: go(k_stage<0, true, false>); break;
+    case 15: go(k_stage<0, true, true>); break;
     case 0: go(k_stage<1, false, false>); break;
     case 1: go(k_stage<1, false, true>); break;
     case 2: go(k_stage<1, true, false>); break;
@@ -400,7 +398,7 @@ int launch_stage(const Ctx &c, int krk, const ws_state &st, ws_soa fl, double dt
     case 9: go(k_stage<3, false, true>); break;
     case 10: go(k_stage<3, true, false>); break;
     case 11: go(k_stage<3, true, true>); break;
-    default: return set_error(WS_ERR_INVALID, "krk must be 1, 2 or 3");
+    default: return set_error(WS_ERR_INVALID, "krk must be 0, 1, 2 or 3");
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "stage launch");

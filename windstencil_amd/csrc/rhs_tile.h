@@ -156,11 +156,14 @@ __device__ __forceinline__ void tile_load(int nyg, const ws_field &d, const ws_f
 // split the loaded points into LDS, call next() (the persistent kernels issue the next
 // tile's loads there, into the registers just consumed), reconstruct the faces, then
 // write the flux difference of every cell.  Every thread of the block calls it.
-template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND, typename Next>
+// With STG the launch is the last of the three and applies the RK3 stage of `sa` to its
+// cells: f = stage(fn, f, du + this direction's terms), du itself is not written.
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND, bool STG, typename Next>
 __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_fields &mf, const ws_soa &du,
                                              const TileGeo &g, TileLoads<DIR, L, W, K> &ld, double *sp,
-                                             Next next)
+                                             const StageArgs &sa, Next next)
 {
+    static_assert(!(STG && (BND || !ACC)), "the stage is applied by an accumulating launch without closures");
     using TL = Tile<DIR, L, W, K>;
     const int t = threadIdx.x;
     const double cA1 = k.A[3 * DIR + 0], cA2 = k.A[3 * DIR + 1], cA3 = k.A[3 * DIR + 2];
@@ -307,6 +310,7 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
 
     // ---- phase 3: flux difference of each cell ---------------------------------------------
     const double scale0 = -k.rh[DIR];
+    const double sdt = STG ? (sa.dt_dev ? *sa.dt_dev : sa.dt) : 0.0;
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         const int c = t + r * TL::NT;
@@ -339,8 +343,25 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
             } else if (last_only_p) {
                 val = scale * nfix_lds[l * 5 + m];
             }
-            if (ACC) dp[m * vstride] = duo[r][m] + val;
-            else dp[m * vstride] = val;
+            if (STG) {
+                const double dtot = duo[r][m] + val;
+                const size_t fo = (size_t)(cx - LAP) +
+                                  (size_t)sa.f.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP)) +
+                                  (size_t)m * sa.f.pitch * k.ny * k.nz;
+                const size_t no = (size_t)(cx - LAP) +
+                                  (size_t)sa.fn.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP)) +
+                                  (size_t)m * sa.fn.pitch * k.ny * k.nz;
+                const double fnv = sa.fn.ptr[no];
+                double rv;
+                if (sa.krk == 1) rv = rk3_stage_value<1>(fnv, 0.0, dtot, sdt);
+                else if (sa.krk == 2) rv = rk3_stage_value<2>(fnv, sa.f.ptr[fo], dtot, sdt);
+                else rv = rk3_stage_value<3>(fnv, sa.f.ptr[fo], dtot, sdt);
+                sa.f.ptr[fo] = rv;
+            } else if (ACC) {
+                dp[m * vstride] = duo[r][m] + val;
+            } else {
+                dp[m * vstride] = val;
+            }
         }
     }
 }
@@ -350,10 +371,10 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
 // dispatch of work-groups to the eight XCDs each XCD gets a contiguous range of tiles, so
 // neighbouring tiles, which share halo lines and the cache lines of their edge columns,
 // meet in the same L2.
-template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND>
+template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND, bool STG = false>
 __global__ void __launch_bounds__(320)
 k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_metric_fields mf,
-           ws_soa du, Box box, int gx, int gy, int ntiles)
+           ws_soa du, Box box, int gx, int gy, int ntiles, StageArgs sa)
 {
     using TL = Tile<DIR, L, W, K>;
     __shared__ double sp[10 * TL::QS + TL::PAD];
@@ -364,7 +385,7 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
     const TileGeo g = tile_geo<DIR, L, W, K>(box, tile - r * gx, r % gy, r / gy);
     TileLoads<DIR, L, W, K> ld;
     tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
-    tile_compute<DIR, L, W, K, FIELDS, ACC, MIX, BND>(k, mf, du, g, ld, sp, [] {});
+    tile_compute<DIR, L, W, K, FIELDS, ACC, MIX, BND, STG>(k, mf, du, g, ld, sp, sa, [] {});
 }
 
 // Persistent form for one box: the resident work-groups each walk a contiguous run of
@@ -396,7 +417,7 @@ k_rhs_persist(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_fiel
         auto next = [&] {
             if (PREFETCH && more) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, gn, ld);
         };
-        tile_compute<DIR, L, W, K, false, ACC, MIX, false>(k, nomf, du, g, ld, sp, next);
+        tile_compute<DIR, L, W, K, false, ACC, MIX, false, false>(k, nomf, du, g, ld, sp, StageArgs{}, next);
         __syncthreads();   // the next tile reuses the LDS buffer
         g = gn;
     }
@@ -455,7 +476,7 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
                 tile_load<DIR, L, W, K>(bn.n[1] + 2 * LAP, bn.d, bn.u, bn.v, bn.w, bn.cc, gn, ld);
             }
         };
-        tile_compute<DIR, L, W, K, false, ACC, MIX, false>(kb, nomf, b.du, g, ld, sp, next);
+        tile_compute<DIR, L, W, K, false, ACC, MIX, false, false>(kb, nomf, b.du, g, ld, sp, StageArgs{}, next);
         __syncthreads();   // the next tile reuses the LDS buffer
         tl = tn;
         g = gn;
@@ -487,7 +508,7 @@ k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int nti
     TileLoads<DIR, L, W, K> ld;
     tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
     const ws_metric_fields nomf{};
-    tile_compute<DIR, L, W, K, false, ACC, MIX, false>(kb, nomf, b.du, g, ld, sp, [] {});
+    tile_compute<DIR, L, W, K, false, ACC, MIX, false, false>(kb, nomf, b.du, g, ld, sp, StageArgs{}, [] {});
 }
 
 // work-groups of a persistent kernel that fit on the device at once
@@ -534,10 +555,10 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     if (k.bnd_any[DIR]) {
         // one-sided closures in this direction: the grid schedule with the closure code
         hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, true>), grid1, dim3(TL::NT), 0, s, k, d, u, v,
-                           w, cc, mf, du, box, (int)grid.x, (int)grid.y, ntiles);
+                           w, cc, mf, du, box, (int)grid.x, (int)grid.y, ntiles, StageArgs{});
     } else if (FIELDS || sched == WS_SCHED_GRID) {
         hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, FIELDS, ACC, MIX, false>), grid1, dim3(TL::NT), 0, s, k, d, u,
-                           v, w, cc, mf, du, box, (int)grid.x, (int)grid.y, ntiles);
+                           v, w, cc, mf, du, box, (int)grid.x, (int)grid.y, ntiles, StageArgs{});
     } else if (sched == WS_SCHED_PREFETCH) {
         launch_persist<DIR, L, W, K, ACC, MIX, true>(k, d, u, v, w, cc, du, box, grid.x, grid.y, grid.z, s);
     } else {
@@ -545,48 +566,80 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     }
 }
 
+// The last launch of an RK3 stage that applies the stage itself (grid schedule, constant
+// metrics, no closures).
+template <int DIR, int L, int W, int K, bool MIX>
+void launch_dir_stage(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_soa du,
+                      const Box &box, const StageArgs &sa, hipStream_t s)
+{
+    using TL = Tile<DIR, L, W, K>;
+    const int nx = box.e[0] - box.s[0], ny = box.e[1] - box.s[1], nz = box.e[2] - box.s[2];
+    const int nxa = box.e[0] - box.s[0] / TL::XALIGN * TL::XALIGN;
+    dim3 grid;
+    if (DIR == 0) grid = dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
+    else if (DIR == 1) grid = dim3((nxa + W - 1) / W, (ny + L - 1) / L, nz);
+    else grid = dim3((nxa + W - 1) / W, (nz + L - 1) / L, ny);
+    const int ntiles = (int)(grid.x * grid.y * grid.z);
+    const ws_metric_fields nomf{};
+    hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, false, true, MIX, false, true>), dim3(8 * ((ntiles + 7) / 8)),
+                       dim3(TL::NT), 0, s, k, d, u, v, w, cc, nomf, du, box, (int)grid.x, (int)grid.y, ntiles, sa);
+}
+
 // The first launch writes du (or accumulates into it when the caller asks), the other two
 // read-modify-write it.  Which direction pays for that read shifts the time differently
-// per precision, so the order is a tuning choice (WS_ORDER_*).
+// per precision, so the order is a tuning choice (WS_ORDER_*).  With a stage (sa) the last
+// launch applies it instead of writing du.
 template <int L0, int W0, int L1, int W1, int L2, int W2, int K, bool FIELDS, bool MIX>
 void launch_all(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                 const ws_metric_fields &mf, ws_soa du, const Box &box, int accumulate, int sched, int order,
-                hipStream_t s)
+                const StageArgs *sa, hipStream_t s)
 {
     if (order == WS_ORDER_ZYX) {
         if (accumulate) launch_dir<2, L2, W2, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
         else launch_dir<2, L2, W2, K, FIELDS, false, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
         launch_dir<1, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+        if constexpr (!FIELDS) {
+            if (sa) {
+                launch_dir_stage<0, L0, W0, K, MIX>(k, d, u, v, w, cc, du, box, *sa, s);
+                return;
+            }
+        }
         launch_dir<0, L0, W0, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
         return;
     }
     if (accumulate) launch_dir<0, L0, W0, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
     else launch_dir<0, L0, W0, K, FIELDS, false, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
     launch_dir<1, L1, W1, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
+    if constexpr (!FIELDS) {
+        if (sa) {
+            launch_dir_stage<2, L2, W2, K, MIX>(k, d, u, v, w, cc, du, box, *sa, s);
+            return;
+        }
+    }
     launch_dir<2, L2, W2, K, FIELDS, true, MIX>(k, d, u, v, w, cc, mf, du, box, sched, s);
 }
 
 template <int S, bool FIELDS, bool MIX>
 void launch_shape_n(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                     const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, int order,
-                    hipStream_t s)
+                    const StageArgs *sa, hipStream_t s)
 {
     constexpr TileShapeDef D = kTileShapes[S];
     launch_all<D.L0, D.W0, D.L1, D.W1, D.L2, D.W2, D.K, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched,
-                                                                    order, s);
+                                                                    order, sa, s);
 }
 
 template <bool FIELDS, bool MIX>
 void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                   const ws_metric_fields &mf, ws_soa du, const Box &box, int acc, int sched, int order,
-                  hipStream_t s)
+                  const StageArgs *sa, hipStream_t s)
 {
     switch (shape) {
-    case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
-    case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
-    case 2: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
-    case 3: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
-    default: launch_shape_n<4, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, s); break;
+    case 0: launch_shape_n<0, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, sa, s); break;
+    case 1: launch_shape_n<1, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, sa, s); break;
+    case 2: launch_shape_n<2, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, sa, s); break;
+    case 3: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, sa, s); break;
+    default: launch_shape_n<4, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, sa, s); break;
     }
 }
 
@@ -633,7 +686,8 @@ void launch_batch_pf(const KParams &k, const BatchTiles &bt, hipStream_t s)
 
 template <bool MIX>
 int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
-                    const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s)
+                    const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s,
+                    const StageArgs *sa = nullptr)
 {
     Box box;
     box.s[0] = (int)job.start.x; box.s[1] = (int)job.start.y; box.s[2] = (int)job.start.z;
@@ -642,9 +696,11 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
     const int shape = c.tile_shape;
     if (m) {
         mf = *m;
-        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order, s);
+        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order,
+                                nullptr, s);
     } else {
-        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order, s);
+        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order, sa,
+                                 s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, MIX ? "mixed-precision RHS launch" : "tiled RHS launch");

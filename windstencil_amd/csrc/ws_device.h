@@ -116,6 +116,23 @@ __device__ __forceinline__ void split_ref(const KParams &k, double d, double u, 
     fm[4] = tmp0 * (E1M*vv + E2M*vvc1 + E3M*vvc2 + W2*(E2M + E3M));
 }
 
+// SSP-RK3 stage value of one conserved component (src/solver/OCFD_time.cpp:15-65) in the
+// reference's order with contraction off: fnv the step's initial state, fv the stage's
+// current f (unused in stage 1), duv the RHS.  Shared by the stage kernels and the RHS
+// launch that applies the stage itself, so both give the same bits.
+template <int KRK>
+__device__ __forceinline__ double rk3_stage_value(double fnv, double fv, double duv, double dt)
+{
+#pragma clang fp contract(off)
+    if (KRK == 1) return fnv + dt * duv;
+    if (KRK == 2) {
+        const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
+        return t1 * fnv + t2 * (fv + dt * duv);
+    }
+    const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
+    return t1 * fnv + t2 * (fv + dt * duv);
+}
+
 // ---------------------------------------------------------------------------
 // Fast forms (tiled kernels)
 // ---------------------------------------------------------------------------

@@ -38,6 +38,16 @@ struct KParams {
     int bnd_any[3];
 };
 
+// The SSP-RK3 stage the last directional RHS launch applies to its cells itself (f from fn,
+// f and the complete RHS, which then never goes through du): stage krk (1..3), dt from
+// dt_dev when that is set.
+struct StageArgs {
+    ws_soa f, fn;
+    const double *dt_dev;
+    double dt;
+    int krk;
+};
+
 struct Ctx {
     ws_config cfg;
     KParams kp;
@@ -54,6 +64,8 @@ struct Ctx {
     int schedule = WS_SCHED_GRID;
     // order of the directional launches (WS_ORDER_*)
     int dir_order = WS_ORDER_XYZ;
+    // RK3 steps apply each stage's update in the last RHS launch where they can
+    int fuse_stage = 1;
     // viscous terms: twelve ghosted flux fields, allocated at the first use
     double *vis_scratch = nullptr;
     size_t vis_scratch_elems = 0;
@@ -95,6 +107,13 @@ int launch_rhs_tiled(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field 
                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
 int launch_rhs_mixed(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
+// the RHS with the RK3 stage `sa` applied by its last launch (grid schedule; the caller
+// checks that the configuration allows it: tiled kernels, constant metrics, no closures,
+// no viscous terms); du holds partial sums afterwards
+int launch_rhs_tiled_stage(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_soa du,
+                           ws_job job, const StageArgs &sa, hipStream_t s);
+int launch_rhs_mixed_stage(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc, ws_soa du,
+                           ws_job job, const StageArgs &sa, hipStream_t s);
 
 int launch_rhs_batch_tiled(const KParams &k, const BatchTiles &bt, hipStream_t s);
 int launch_rhs_batch_mixed(const KParams &k, const BatchTiles &bt, hipStream_t s);
