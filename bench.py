@@ -60,8 +60,8 @@ def parse():
     ap.add_argument("--rk3-steps", type=int, default=0, help="RK3 steps of the batched/rk3 workloads "
                                                             "(default: 10 batched, 100 rk3)")
     ap.add_argument("--stage-fusion", type=int, default=1, choices=[0, 1],
-                    help="rk3 workload: 1 lets the last RHS launch apply each stage (default), 0 the "
-                         "separate stage kernel")
+                    help="rk3 and batched RK3: 1 lets the last RHS launch apply each stage (default), 0 "
+                         "the separate stage kernel")
     ap.add_argument("--probe", action="store_true",
                     help="only measure the device's VALU rates (ws_probe_valu) and print them")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -243,6 +243,7 @@ def bench_batched(args):
                                  W.job((0, 0, 0), (nx, ny, nz)), sh))
         states.append(st)
     b = W.Batch(W.default_config(8, 8, 8, precision=prec), extents, states)
+    W.check(L.ws_batch_set_stage_fusion(b.handle, args.stage_fusion))
     N = b.cells()
     W.check(L.ws_batch_du_comput(b.handle, sh))
     torch.cuda.synchronize()

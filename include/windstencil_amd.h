@@ -395,6 +395,9 @@ int ws_batch_set_kernel(ws_batch *batch, int32_t kernel);
 int ws_batch_set_tile_shape(ws_batch *batch, int32_t shape);
 int ws_batch_get_tile_shape(const ws_batch *batch, int32_t *shape);
 int ws_batch_set_schedule(ws_batch *batch, int32_t schedule);
+/* As ws_ctx_set_stage_fusion, for the batch's RK3 steps (the batched z launch applies the
+ * stage with the grid schedule and no viscous terms). */
+int ws_batch_set_stage_fusion(ws_batch *batch, int32_t on);
 int ws_batch_get_schedule(const ws_batch *batch, int32_t *schedule);
 /* RHS of every block from ghost-filled d,u,v,w and cc (overwrites du). */
 int ws_batch_inviscid_rhs(ws_batch *batch, hipStream_t stream);

@@ -104,11 +104,15 @@ def _to_cons(ctx, st):
                              W.job((0, 0, 0), (nx, ny, nz)), W.stream_handle()))
 
 
+@pytest.mark.parametrize("schedule", [W.WS_SCHED_GRID, W.WS_SCHED_PREFETCH])
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
-def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision):
+def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision, schedule):
+    """With the grid schedule the batched z launch applies each RK3 stage itself; with the
+    prefetching schedule the stage kernels do: both match single-box runs."""
     extents = _extents(13, 5, 6, 33)
     dt = 2e-4
     b, prims = _batch(extents, 500, precision=precision, dt=dt)
+    W.check(W.lib().ws_batch_set_schedule(b.handle, schedule))
     for e, st in zip(extents, b.states):
         _to_cons(make_ctx(*e), st)
     log(f"batch rk3 fixed dt {extents}")
@@ -124,12 +128,14 @@ def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision):
         assert rel_l2(st.interior("f").cpu().numpy(), ref.interior("f").cpu().numpy()) <= 1e-12, e
 
 
+@pytest.mark.parametrize("schedule", [W.WS_SCHED_GRID, W.WS_SCHED_PREFETCH])
 @pytest.mark.parametrize("nsteps", [2, 3, 5])
-def test_batch_graph_replay_equals_eager(ws, nsteps):
+def test_batch_graph_replay_equals_eager(ws, nsteps, schedule):
     extents = _extents(17, 4, 6, 28)
     runs = []
     for graph in (1, 0):
         b, _ = _batch(extents, 600)
+        W.check(W.lib().ws_batch_set_schedule(b.handle, schedule))
         for e, st in zip(extents, b.states):
             _to_cons(make_ctx(*e), st)
         W.check(W.lib().ws_batch_rk3_run(b.handle, nsteps, 0.5, graph, W.stream_handle()))

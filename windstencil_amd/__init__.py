@@ -153,6 +153,7 @@ _SIGS = {
     "ws_batch_set_tile_shape": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_batch_get_tile_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_batch_set_schedule": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_batch_set_stage_fusion": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_batch_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_batch_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ws_batch_du_comput": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

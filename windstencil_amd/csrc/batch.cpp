@@ -29,6 +29,8 @@ struct ws_batch {
     std::vector<ws_ctx *> ctx;
     std::vector<ws_state> st;
     BatchBlock *d_blocks = nullptr;
+    std::vector<double *> fas, fbs;   // each block's f and fn as created (the table's fa, fb)
+    double *fb0 = nullptr;
     int4 *d_tiles[WS_TILE_COUNT][3] = {};
     int ntiles[WS_TILE_COUNT][3] = {};
     double *d_dts = nullptr;    // per-block CFL steps
@@ -68,6 +70,7 @@ BatchTiles tiles_of(const ws_batch *b)
     t.shape = b->tile_shape;
     t.schedule = b->schedule;
     t.blocks = b->d_blocks;
+    t.stage = nullptr;
     for (int d = 0; d < 3; ++d) {
         t.tiles[d] = b->d_tiles[b->tile_shape][d];
         t.ntiles[d] = b->ntiles[b->tile_shape][d];
@@ -75,7 +78,7 @@ BatchTiles tiles_of(const ws_batch *b)
     return t;
 }
 
-int batch_rhs(ws_batch *b, hipStream_t s)
+int batch_rhs(ws_batch *b, hipStream_t s, const BatchStage *stage = nullptr)
 {
     if (b->kernel == WS_KERNEL_ANCHOR) {
         // the reference-order kernels have no batched form: one RHS per block
@@ -86,7 +89,8 @@ int batch_rhs(ws_batch *b, hipStream_t s)
         return WS_OK;
     }
     const KParams &k = b->ctx[0]->kp;   // physics constants and metrics are the batch's
-    const BatchTiles t = tiles_of(b);
+    BatchTiles t = tiles_of(b);
+    t.stage = stage;
     return b->precision == WS_PREC_MIXED ? launch_rhs_batch_mixed(k, t, s) : launch_rhs_batch_tiled(k, t, s);
 }
 
@@ -124,6 +128,11 @@ int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *exten
         std::memset(&h, 0, sizeof(h));
         h.d = states[i].d; h.u = states[i].u; h.v = states[i].v; h.w = states[i].w; h.cc = states[i].cc;
         h.du = states[i].du;
+        h.fa = states[i].f;
+        h.fb = states[i].fn;
+        b->fas.push_back(states[i].f.ptr);
+        b->fbs.push_back(states[i].fn.ptr);
+        if (i == 0) b->fb0 = states[i].fn.ptr;
         h.n[0] = cfg.nx; h.n[1] = cfg.ny; h.n[2] = cfg.nz;
         h.rh[0] = 1.0 / cfg.hx; h.rh[1] = 1.0 / cfg.hy; h.rh[2] = 1.0 / cfg.hz;
         b->cells += (long long)cfg.nx * cfg.ny * cfg.nz;
@@ -205,6 +214,13 @@ int ws_batch_set_tile_shape(ws_batch *b, int32_t shape)
     return WS_OK;
 }
 
+int ws_batch_set_stage_fusion(ws_batch *b, int32_t on)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    for (ws_ctx *c : b->ctx) c->fuse_stage = on ? 1 : 0;
+    return WS_OK;
+}
+
 int ws_batch_get_tile_shape(const ws_batch *b, int32_t *shape)
 {
     if (!b || !shape) return set_error(WS_ERR_INVALID, "null argument");
@@ -266,11 +282,21 @@ int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
     for (ws_state &st : b->st) std::swap(st.f, st.fn);
     const ws_soa nolap{};
     int r = WS_OK;
+    // With the tiled kernels, the grid schedule and no viscous terms the batched z launch
+    // applies each stage (as a single box's RK3 step does); the block table holds every
+    // block's f and fn as created, and whether they are exchanged now is the same for all
+    // blocks (they swap together).
+    bool fuse = b->kernel == WS_KERNEL_TILED && !b->base.viscous && b->schedule == WS_SCHED_GRID &&
+                b->ctx[0]->fuse_stage;
+    const int swapped = b->st[0].f.ptr == b->fb0 ? 1 : 0;
+    for (size_t i = 0; i < b->st.size() && fuse; ++i)
+        fuse = b->st[i].f.ptr == (swapped ? b->fbs[i] : b->fas[i]);
     // the blocks are periodic (ws_batch_create); stages 1 and 2 leave the next stage's
     // inputs prepared (fused stage kernel writing the sound speed, then one ghost fill)
     for (int krk = 1; krk <= 3 && !r; ++krk) {
         for (size_t i = 0; i < b->ctx.size() && !r && krk == 1; ++i) r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
-        if (!r) r = batch_rhs(b, s);
+        const BatchStage bs{dt_dev, b->base.dt, krk, swapped};
+        if (!r) r = batch_rhs(b, s, fuse ? &bs : nullptr);
         for (size_t i = 0; i < b->ctx.size() && !r && b->base.viscous; ++i) {
             const ws_state &st = b->st[i];
             r = launch_viscous(*b->ctx[i], st.u, st.v, st.w, st.T, nullptr, st.du, s);
@@ -278,7 +304,7 @@ int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
         for (size_t i = 0; i < b->ctx.size() && !r; ++i) {
             ws_ctx *c = b->ctx[i];
             const ws_state &st = b->st[i];
-            r = launch_stage(*c, krk, st, nolap, c->cfg.dt, dt_dev, c->cfg.bc_planes != 0, krk < 3, s);
+            r = launch_stage(*c, fuse ? 0 : krk, st, nolap, c->cfg.dt, dt_dev, c->cfg.bc_planes != 0, krk < 3, s);
             if (!r && krk < 3) {
                 const ws_field g[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
                 r = launch_ghost_fill(*c, g, 6, s);

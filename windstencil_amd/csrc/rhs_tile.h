@@ -485,9 +485,9 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
 
 // Batched form with one tile per work-group (grid = tile count): no loop-carried
 // state, so it needs no more registers than the single-box kernel.
-template <int DIR, int L, int W, int K, bool ACC, bool MIX>
+template <int DIR, int L, int W, int K, bool ACC, bool MIX, bool STG = false>
 __global__ void __launch_bounds__(320)
-k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles)
+k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, BatchStage bs)
 {
     using TL = Tile<DIR, L, W, K>;
     __shared__ double sp[10 * TL::QS + TL::PAD];
@@ -508,7 +508,9 @@ k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int nti
     TileLoads<DIR, L, W, K> ld;
     tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
     const ws_metric_fields nomf{};
-    tile_compute<DIR, L, W, K, false, ACC, MIX, false, false>(kb, nomf, b.du, g, ld, sp, StageArgs{}, [] {});
+    StageArgs sa{};
+    if (STG) sa = StageArgs{bs.swapped ? b.fb : b.fa, bs.swapped ? b.fa : b.fb, bs.dt_dev, bs.dt, bs.krk};
+    tile_compute<DIR, L, W, K, false, ACC, MIX, false, STG>(kb, nomf, b.du, g, ld, sp, sa, [] {});
 }
 
 // work-groups of a persistent kernel that fit on the device at once
@@ -650,9 +652,17 @@ void launch_batch_dir(const KParams &k, const BatchTiles &bt, hipStream_t s)
 {
     const int n = bt.ntiles[DIR];
     if (n <= 0) return;
+    if constexpr (DIR == 2 && ACC) {
+        if (bt.stage) {
+            // the last launch applies the RK3 stage (grid schedule only: the caller checks)
+            hipLaunchKernelGGL((k_rhs_batch_grid<DIR, L, W, K, true, MIX, true>), dim3(8 * ((n + 7) / 8)),
+                               dim3(Tile<DIR, L, W, K>::NT), 0, s, k, bt.blocks, bt.tiles[DIR], n, *bt.stage);
+            return;
+        }
+    }
     if (bt.schedule == WS_SCHED_GRID) {
         hipLaunchKernelGGL((k_rhs_batch_grid<DIR, L, W, K, ACC, MIX>), dim3(8 * ((n + 7) / 8)),
-                           dim3(Tile<DIR, L, W, K>::NT), 0, s, k, bt.blocks, bt.tiles[DIR], n);
+                           dim3(Tile<DIR, L, W, K>::NT), 0, s, k, bt.blocks, bt.tiles[DIR], n, BatchStage{});
         return;
     }
     static const int resident = resident_of(k_rhs_batch<DIR, L, W, K, ACC, MIX, PF>, Tile<DIR, L, W, K>::NT);

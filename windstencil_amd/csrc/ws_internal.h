@@ -77,9 +77,19 @@ KParams make_kparams(const ws_config &cfg);
 struct BatchBlock {
     ws_field d, u, v, w, cc;
     ws_soa du;
+    ws_soa fa, fb;       // the block's f and fn as created (RK3 steps exchange them)
     int n[3];            // interior extents
     int pad;
     double rh[3];        // 1/h per direction
+};
+
+// The RK3 stage the last batched RHS launch applies: every block's f is fb and fn fa when
+// `swapped`, else f = fa and fn = fb.
+struct BatchStage {
+    const double *dt_dev;
+    double dt;
+    int krk;
+    int swapped;
 };
 
 // Device tile lists of a batch for one tile shape: entry (block, bx, by, bz) per tile,
@@ -90,6 +100,7 @@ struct BatchTiles {
     const BatchBlock *blocks;
     const int4 *tiles[3];
     int ntiles[3];
+    const BatchStage *stage;   // non-null: the z launch applies this RK3 stage (grid schedule)
 };
 
 // (tile shape, schedule, direction order) the autotuner found best for these extents and
