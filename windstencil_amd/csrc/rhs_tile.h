@@ -9,7 +9,9 @@
 // line), into LDS: 10 doubles per point.  Phase 2 gives each thread K consecutive
 // faces of one line; for every variable it reconstructs them with the positive and
 // the negative rule (sharing differences between the K faces) and merges each pair.
-// Phase 3 turns face fluxes into cell updates.
+// Phase 3 turns face fluxes into cell updates: the first of the three launches writes
+// du, the other two accumulate into it, and in an RK3 step the last one may instead
+// apply the stage to f (STG).
 // MIX selects the reconstruction: FP64 (weno_k, the pair merged over one reciprocal)
 // or FP32 relative to FP64 base points (weno_k_mixed); splitting, the merge and the
 // flux difference are FP64 either way.
