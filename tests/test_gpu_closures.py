@@ -1,7 +1,7 @@
 """GPU parity of the one-sided boundary closures (the reference's D0_bound path) and the
 non-reflecting flags: non-periodic directions keep the caller's ghost layers (filled here
 with arbitrary valid values, given identically to the oracle).  Anchor kernels bitwise,
-tiled FP64 within 1e-12 for every tile shape, mixed within 1e-6."""
+tiled FP64 within 1e-12 for every tile shape and direction order, mixed within 1e-6."""
 import numpy as np
 import pytest
 
@@ -70,13 +70,17 @@ def test_closures_match_oracle(orc, ws, name, periodic, d0, nr):
     log(f"closures {name}: anchor max |diff| {np.abs(da - ref).max():.3e}")
     assert np.array_equal(da, ref)
     for tile in W.TILE_SHAPES:
-        tctx = make_ctx(nx, ny, nz, **kw)
-        tctx.set_tile_shape(tile)
-        err = rel_l2(gpu_du_comput(tctx, _upload(g)), ref)
-        log(f"  tile {tile}: rel L2 {err:.3e}")
-        assert err <= 1e-12
-    mctx = make_ctx(nx, ny, nz, precision=W.WS_PREC_MIXED, **kw)
-    assert rel_l2(gpu_du_comput(mctx, _upload(g)), ref) <= 1e-6
+        for order in W.ORDERS:
+            tctx = make_ctx(nx, ny, nz, **kw)
+            tctx.set_tile_shape(tile)
+            tctx.set_direction_order(order)
+            err = rel_l2(gpu_du_comput(tctx, _upload(g)), ref)
+            log(f"  tile {tile} order {order}: rel L2 {err:.3e}")
+            assert err <= 1e-12
+    for order in W.ORDERS:
+        mctx = make_ctx(nx, ny, nz, precision=W.WS_PREC_MIXED, **kw)
+        mctx.set_direction_order(order)
+        assert rel_l2(gpu_du_comput(mctx, _upload(g)), ref) <= 1e-6
 
 
 def test_closures_on_a_sub_box_match_the_full_box(orc, ws):

@@ -203,13 +203,15 @@ def test_metric_fields(orc, ws):
             assert rel_l2(du, ref) <= TOL_FP64
 
 
-def test_sub_box_job_and_accumulate(ws):
+@pytest.mark.parametrize("order", W.ORDERS)
+def test_sub_box_job_and_accumulate(ws, order):
     """A job box inside the interior updates exactly its cells, with the values of a
-    full-domain run; accumulate = 1 adds to what du held."""
+    full-domain run; accumulate = 1 adds to what du held (both direction orders)."""
     import torch
     nx, ny, nz = 40, 24, 20
     prims = random_state(nx, ny, nz, 3, "shock")
     ctx = make_ctx(nx, ny, nz)
+    ctx.set_direction_order(order)
     full = gpu_du_comput(ctx, upload_state(prims))
     st = upload_state(prims)
     st.t["du"].fill_(7.0)
