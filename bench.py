@@ -59,9 +59,10 @@ def parse():
                          "rk3: SSP-RK3 steps of one box in a graph")
     ap.add_argument("--rk3-steps", type=int, default=0, help="RK3 steps of the batched/rk3 workloads "
                                                             "(default: 10 batched, 100 rk3)")
-    ap.add_argument("--stage-fusion", type=int, default=1, choices=[0, 1],
-                    help="rk3 and batched RK3: 1 lets the last RHS launch apply each stage (default), 0 "
-                         "the separate stage kernel")
+    ap.add_argument("--stage-fusion", type=int, default=2, choices=[0, 1, 2],
+                    help="rk3 and batched RK3: 2 (default) lets the last RHS launch apply each stage and, "
+                         "for one box, derive the primitives of stages 1 and 2; 1 the stage only; 0 the "
+                         "separate stage kernel")
     ap.add_argument("--probe", action="store_true",
                     help="only measure the device's VALU rates (ws_probe_valu) and print them")
     ap.add_argument("--calibrate-copy", action="store_true",

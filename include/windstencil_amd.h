@@ -183,9 +183,12 @@ int ws_ctx_get_schedule(const ws_ctx *ctx, int32_t *schedule);
  * precision (the autotuner picks) */
 int ws_ctx_set_direction_order(ws_ctx *ctx, int32_t order);
 int ws_ctx_get_direction_order(const ws_ctx *ctx, int32_t *order);
-/* 1 (default): RK3 steps let the last directional RHS launch apply each stage's update
+/* 2 (default): RK3 steps let the last directional RHS launch apply each stage's update
  * (tiled kernels, grid schedule, constant metrics, no closures, no viscous terms; same
- * result as the separate stage kernel, du then holds partial sums); 0: separate update. */
+ * result as the separate stage kernel, du then holds partial sums) and, with every
+ * direction periodic, derive the primitives of stages 1 and 2 in that launch too (into a
+ * second set of six ghosted fields the context allocates at the first eager step; same
+ * state bits at the end of the step); 1: the update only; 0: separate update. */
 int ws_ctx_set_stage_fusion(ws_ctx *ctx, int32_t on);
 
 /* Allocate / free the solver state with the reference's extents
@@ -395,8 +398,9 @@ int ws_batch_set_kernel(ws_batch *batch, int32_t kernel);
 int ws_batch_set_tile_shape(ws_batch *batch, int32_t shape);
 int ws_batch_get_tile_shape(const ws_batch *batch, int32_t *shape);
 int ws_batch_set_schedule(ws_batch *batch, int32_t schedule);
-/* As ws_ctx_set_stage_fusion, for the batch's RK3 steps (the batched z launch applies the
- * stage with the grid schedule and no viscous terms). */
+/* As ws_ctx_set_stage_fusion, for the batch's RK3 steps: nonzero lets the batched z launch
+ * apply the stage (grid schedule, no viscous terms); the primitives of a batch are always
+ * derived by the stage kernel. */
 int ws_batch_set_stage_fusion(ws_batch *batch, int32_t on);
 int ws_batch_get_schedule(const ws_batch *batch, int32_t *schedule);
 /* RHS of every block from ghost-filled d,u,v,w and cc (overwrites du). */

@@ -172,20 +172,25 @@ def test_rk3_tiled_matches_oracle(orc, ws, kind):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("fusion", [1, 2])
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
 @pytest.mark.parametrize("order", [W.WS_ORDER_XYZ, W.WS_ORDER_ZYX])
 @pytest.mark.parametrize("graph", [False, True])
-def test_stage_applied_by_the_last_rhs_launch_equals_the_stage_kernel(ws, precision, order, graph):
-    """RK3 steps whose last directional RHS launch applies the stage (the default) against
-    the separate stage kernel reading du back: the same stage expression on the same
-    complete RHS, so the states agree bit for bit (boundary planes on, CFL step from the
-    device)."""
+def test_stage_applied_by_the_last_rhs_launch_equals_the_stage_kernel(ws, precision, order, graph, fusion):
+    """RK3 steps whose last directional RHS launch applies the stage (fusion 1), and also
+    derives the primitives of stages 1 and 2 into the context's second primitive set
+    (fusion 2, the default), against the separate stage kernel reading du back: the same
+    stage and primitive expressions on the same complete RHS, so the states -- f and every
+    ghosted primitive field, ghost layers included -- agree bit for bit (boundary planes
+    on, CFL step from the device)."""
     prims = random_state(45, 26, 19, 13, "shock")
-    a, _ = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=True)
-    b, _ = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=False)
-    log(f"  fused vs separate stage: prec {precision} order {order} graph {graph}: max |diff| "
+    a, sa = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=fusion)
+    b, sb = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=0)
+    log(f"  fusion {fusion} vs separate stage: prec {precision} order {order} graph {graph}: max |diff| "
         f"{np.abs(a - b).max():.3e}, rel L2 {rel_l2(a, b):.3e}")
     assert np.array_equal(a, b)
+    for k in ("d", "u", "v", "w", "T", "P", "cc"):
+        assert np.array_equal(sa.t[k].cpu().numpy(), sb.t[k].cpu().numpy()), k
 
 
 @pytest.mark.parametrize("nsteps", [2, 3, 4, 5])

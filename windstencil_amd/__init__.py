@@ -263,8 +263,10 @@ class Context:
     def set_direction_order(self, o):
         check(lib().ws_ctx_set_direction_order(self.handle, o), "ws_ctx_set_direction_order")
 
-    def set_stage_fusion(self, on):
-        check(lib().ws_ctx_set_stage_fusion(self.handle, 1 if on else 0), "ws_ctx_set_stage_fusion")
+    def set_stage_fusion(self, level):
+        """0: separate stage kernel; 1: the last RHS launch applies each stage; 2 (default):
+        it also derives the primitives of stages 1 and 2."""
+        check(lib().ws_ctx_set_stage_fusion(self.handle, int(level)), "ws_ctx_set_stage_fusion")
 
 
 # ---- torch-backed buffers ---------------------------------------------------------------------

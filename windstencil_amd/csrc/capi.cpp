@@ -236,6 +236,35 @@ int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s)
 // value from fn, f and the complete RHS in registers, the same expression as the stage
 // kernel) and the stage kernel then only derives the primitives: du never carries the
 // complete RHS, which saves its write and read-back.
+//
+// When moreover every direction is periodic, the last launch of stages 1 and 2 also derives
+// the primitives and the sound speed itself (the stage kernel's expressions) and writes them
+// to the context's second primitive set: stage 1 reads the state's fields and writes the
+// second set, stage 2 reads the second set and writes the state's d, u, v, w, T and cc,
+// and stage 3 ends with the stage kernel (primitives, P and the boundary planes of the
+// state).  A ghost fill of the six fields follows stages 1 and 2, so the state ends the
+// step with the bits of the unfused step.  The second set is allocated by the first eager
+// step; a step captured before that runs without it.
+int prim_set(ws_ctx *c, hipStream_t s, ws_field out[6])
+{
+    if (!c->prim_scratch) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+        ws_field f{};
+        if (alloc_field(&f, c->kp.nx, c->kp.ny, c->kp.nz) != WS_OK) return 0;   // one field's size
+        hipFree(f.ptr);
+        const size_t per = (size_t)f.pitch * c->kp.nyg * c->kp.nzg;
+        hipError_t e = hipMalloc(&c->prim_scratch, 6 * per * sizeof(double));
+        if (e != hipSuccess) { c->prim_scratch = nullptr; return 0; }
+        e = hipMemset(c->prim_scratch, 0, 6 * per * sizeof(double));
+        if (e != hipSuccess) { hipFree(c->prim_scratch); c->prim_scratch = nullptr; return 0; }
+        c->prim_pitch = f.pitch;
+    }
+    const size_t per = (size_t)c->prim_pitch * c->kp.nyg * c->kp.nzg;
+    for (int q = 0; q < 6; ++q) out[q] = ws_field{c->prim_scratch + q * per, c->prim_pitch, 0};
+    return 1;
+}
+
 int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
 {
     std::swap(st->f, st->fn);
@@ -244,12 +273,31 @@ int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
     const bool closures = c->kp.bnd_any[0] || c->kp.bnd_any[1] || c->kp.bnd_any[2];
     const bool fuse = c->fuse_stage && c->cfg.kernel == WS_KERNEL_TILED && !c->cfg.viscous &&
                       c->schedule == WS_SCHED_GRID && !closures;
+    ws_field second[6];
+    const bool prims = fuse && periodic && c->fuse_stage > 1 && prim_set(c, s, second);
     const ws_field g[6] = {st->d, st->u, st->v, st->w, st->T, st->cc};
+    const bool planes = c->cfg.bc_planes != 0;
     int r = WS_OK;
     for (int krk = 1; krk <= 3 && !r; ++krk) {
         if (krk == 1 || !periodic) r = prepare_rhs_inputs(c, st, s);
         const bool sound = periodic && krk < 3;
-        const bool planes = c->cfg.bc_planes != 0;
+        if (prims) {
+            // inputs and outputs of this stage's RHS: the state's fields or the second set
+            const ws_field *in = krk == 2 ? second : g;
+            const ws_field *out = krk == 1 ? second : g;
+            StageArgs sa{st->f, st->fn, dt_dev, c->cfg.dt, krk};
+            if (krk < 3) {
+                sa.planes = planes;
+                sa.pd = out[0]; sa.pu = out[1]; sa.pv = out[2]; sa.pw = out[3]; sa.pT = out[4]; sa.pcc = out[5];
+            }
+            if (!r)
+                r = c->cfg.precision == WS_PREC_MIXED
+                        ? launch_rhs_mixed_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s)
+                        : launch_rhs_tiled_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s);
+            if (!r && krk == 3) r = launch_stage(*c, 0, *st, nolap, c->cfg.dt, dt_dev, planes, false, s);
+            if (!r && krk < 3) r = launch_ghost_fill(*c, out, 6, s);
+            continue;
+        }
         if (fuse) {
             const StageArgs sa{st->f, st->fn, dt_dev, c->cfg.dt, krk};
             if (!r)
@@ -341,10 +389,9 @@ int ws_ctx_destroy(ws_ctx *c)
     if (!c) return WS_OK;
     if (c->capture_stream) hipStreamDestroy(c->capture_stream);
     if (c->cfl_partial) hipFree(c->cfl_partial);
-    if (c->vis_scratch) {
-        hipDeviceSynchronize();
-        hipFree(c->vis_scratch);
-    }
+    if (c->vis_scratch || c->prim_scratch) hipDeviceSynchronize();
+    if (c->vis_scratch) hipFree(c->vis_scratch);
+    if (c->prim_scratch) hipFree(c->prim_scratch);
     delete c;
     return WS_OK;
 }
@@ -419,7 +466,8 @@ int ws_ctx_get_direction_order(const ws_ctx *c, int32_t *order)
 int ws_ctx_set_stage_fusion(ws_ctx *c, int32_t on)
 {
     if (!c) return set_error(WS_ERR_INVALID, "null context");
-    c->fuse_stage = on ? 1 : 0;
+    if (on < 0 || on > 2) return set_error(WS_ERR_INVALID, "stage fusion must be 0, 1 or 2");
+    c->fuse_stage = on;
     return WS_OK;
 }
 

@@ -193,27 +193,17 @@ __global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws
             fl.ptr[(size_t)(x + LAP) + (size_t)fl.pitch * ((size_t)(y + LAP) + (size_t)(k.ny + 2 * LAP) *
                    ((size_t)(z + LAP) + (size_t)(k.nz + 2 * LAP) * m))] = r[m];
     }
-    const double d1 = r[0];
-    const double u1 = r[1] / d1, v1 = r[2] / d1, w1 = r[3] / d1;
-    double T1 = r[4];
-    T1 = T1 - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1);
-    const double TT = T1 / (d1 * k.cv);
+    const CellPrims p = cell_primitives(r, k.cv);
     const int gx = x + LAP, gy = y + LAP, gz = z + LAP;
-    d.ptr[fidx(d.pitch, k.nyg, gx, gy, gz)] = d1;
-    u.ptr[fidx(u.pitch, k.nyg, gx, gy, gz)] = u1;
-    v.ptr[fidx(v.pitch, k.nyg, gx, gy, gz)] = v1;
-    w.ptr[fidx(w.pitch, k.nyg, gx, gy, gz)] = w1;
-    T.ptr[fidx(T.pitch, k.nyg, gx, gy, gz)] = TT;
-    if (P.ptr) P.ptr[fidx(P.pitch, k.nyg, gx, gy, gz)] = T1 / d1 * (k.gamma - 1.0);
-    if (SOUND) cc.ptr[fidx(cc.pitch, k.nyg, gx, gy, gz)] = sqrt(TT) / k.ama;
+    d.ptr[fidx(d.pitch, k.nyg, gx, gy, gz)] = p.d;
+    u.ptr[fidx(u.pitch, k.nyg, gx, gy, gz)] = p.u;
+    v.ptr[fidx(v.pitch, k.nyg, gx, gy, gz)] = p.v;
+    w.ptr[fidx(w.pitch, k.nyg, gx, gy, gz)] = p.w;
+    T.ptr[fidx(T.pitch, k.nyg, gx, gy, gz)] = p.T;
+    if (P.ptr) P.ptr[fidx(P.pitch, k.nyg, gx, gy, gz)] = p.e / p.d * (k.gamma - 1.0);
+    if (SOUND) cc.ptr[fidx(cc.pitch, k.nyg, gx, gy, gz)] = sqrt(p.T) / k.ama;
     const bool plane = PLANES && (x == 0 || x == k.nx - 1 || y == 0 || y == k.ny - 1 || z == 0 || z == k.nz - 1);
-    if (plane) {
-        r[0] = d1;
-        r[1] = u1 * d1;
-        r[2] = v1 * d1;
-        r[3] = w1 * d1;
-        r[4] = d1 * ((u1 * u1 + v1 * v1 + w1 * w1) * 0.5 + k.cv * TT);
-    }
+    if (plane) plane_conserved(p, k.cv, r);
     if (KRK != 0 || plane)
 #pragma unroll
         for (int m = 0; m < 5; ++m) f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] = r[m];

@@ -333,6 +333,7 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
         // non-reflecting flags: the first cell takes only the M pass, the last only the P pass
         const bool first_only_m = BND && nr_lo && gl0 + i == 0;
         const bool last_only_p = BND && nr_hi && gl0 + i == n_dom - 1;
+        double rv[5];
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             double val = scale * (sp[m * W * TL::F + fr] - sp[m * W * TL::F + fl]);
@@ -354,16 +355,34 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                                   (size_t)sa.fn.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP)) +
                                   (size_t)m * sa.fn.pitch * k.ny * k.nz;
                 const double fnv = sa.fn.ptr[no];
-                double rv;
-                if (sa.krk == 1) rv = rk3_stage_value<1>(fnv, 0.0, dtot, sdt);
-                else if (sa.krk == 2) rv = rk3_stage_value<2>(fnv, sa.f.ptr[fo], dtot, sdt);
-                else rv = rk3_stage_value<3>(fnv, sa.f.ptr[fo], dtot, sdt);
-                sa.f.ptr[fo] = rv;
+                if (sa.krk == 1) rv[m] = rk3_stage_value<1>(fnv, 0.0, dtot, sdt);
+                else if (sa.krk == 2) rv[m] = rk3_stage_value<2>(fnv, sa.f.ptr[fo], dtot, sdt);
+                else rv[m] = rk3_stage_value<3>(fnv, sa.f.ptr[fo], dtot, sdt);
             } else if (ACC) {
                 dp[m * vstride] = duo[r][m] + val;
             } else {
                 dp[m * vstride] = val;
             }
+        }
+        if (STG) {
+            // the new f's primitives for the next stage's RHS (and OCFD_bc on the planes)
+            if (sa.pd.ptr) {
+                const CellPrims p = cell_primitives(rv, k.cv);
+                sa.pd.ptr[fidx(sa.pd.pitch, k.nyg, cx, cy, cz)] = p.d;
+                sa.pu.ptr[fidx(sa.pu.pitch, k.nyg, cx, cy, cz)] = p.u;
+                sa.pv.ptr[fidx(sa.pv.pitch, k.nyg, cx, cy, cz)] = p.v;
+                sa.pw.ptr[fidx(sa.pw.pitch, k.nyg, cx, cy, cz)] = p.w;
+                sa.pT.ptr[fidx(sa.pT.pitch, k.nyg, cx, cy, cz)] = p.T;
+                sa.pcc.ptr[fidx(sa.pcc.pitch, k.nyg, cx, cy, cz)] = sqrt(p.T) / k.ama;
+                const int ix = cx - LAP, iy = cy - LAP, iz = cz - LAP;
+                if (sa.planes && (ix == 0 || ix == k.nx - 1 || iy == 0 || iy == k.ny - 1 || iz == 0 || iz == k.nz - 1))
+                    plane_conserved(p, k.cv, rv);
+            }
+            const size_t fo = (size_t)(cx - LAP) +
+                              (size_t)sa.f.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
+            const size_t fstride = (size_t)sa.f.pitch * k.ny * k.nz;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) sa.f.ptr[fo + m * fstride] = rv[m];
         }
     }
 }

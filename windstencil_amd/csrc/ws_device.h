@@ -133,6 +133,38 @@ __device__ __forceinline__ double rk3_stage_value(double fnv, double fv, double 
     return t1 * fnv + t2 * (fv + dt * duv);
 }
 
+// Primitives of one cell's conserved values r (get_duvwT, src/kernels/commen_kernel.cpp:78-110)
+// with contraction off: e is the internal energy per volume, T = e / (d Cv).  Shared by the
+// stage kernel and the RHS launch that derives the primitives itself.
+struct CellPrims {
+    double d, u, v, w, e, T;
+};
+
+__device__ __forceinline__ CellPrims cell_primitives(const double r[5], double cv)
+{
+#pragma clang fp contract(off)
+    CellPrims p;
+    p.d = r[0];
+    p.u = r[1] / p.d;
+    p.v = r[2] / p.d;
+    p.w = r[3] / p.d;
+    p.e = r[4] - 0.5 * p.d * (p.u * p.u + p.v * p.v + p.w * p.w);
+    p.T = p.e / (p.d * cv);
+    return p;
+}
+
+// Conserved values re-derived from the primitives on a boundary plane (OCFD_bc,
+// src/boundary/OCFD_boundary.cpp:18-92), contraction off.
+__device__ __forceinline__ void plane_conserved(const CellPrims &p, double cv, double r[5])
+{
+#pragma clang fp contract(off)
+    r[0] = p.d;
+    r[1] = p.u * p.d;
+    r[2] = p.v * p.d;
+    r[3] = p.w * p.d;
+    r[4] = p.d * ((p.u * p.u + p.v * p.v + p.w * p.w) * 0.5 + cv * p.T);
+}
+
 // ---------------------------------------------------------------------------
 // Fast forms (tiled kernels)
 // ---------------------------------------------------------------------------

@@ -40,12 +40,17 @@ struct KParams {
 
 // The SSP-RK3 stage the last directional RHS launch applies to its cells itself (f from fn,
 // f and the complete RHS, which then never goes through du): stage krk (1..3), dt from
-// dt_dev when that is set.
+// dt_dev when that is set.  With pd.ptr set the launch also writes the primitives d, u, v,
+// w, T and the sound speed of the new f to the interior cells of pd, pu, pv, pw, pT, pcc
+// (fields that no launch of the same RHS reads) and, with planes, re-derives f on the
+// boundary planes from them.
 struct StageArgs {
     ws_soa f, fn;
     const double *dt_dev;
     double dt;
     int krk;
+    int planes = 0;
+    ws_field pd{}, pu{}, pv{}, pw{}, pT{}, pcc{};
 };
 
 struct Ctx {
@@ -64,11 +69,16 @@ struct Ctx {
     int schedule = WS_SCHED_GRID;
     // order of the directional launches (WS_ORDER_*)
     int dir_order = WS_ORDER_XYZ;
-    // RK3 steps apply each stage's update in the last RHS launch where they can
-    int fuse_stage = 1;
+    // RK3 steps apply each stage's update in the last RHS launch where they can (1), and
+    // there also derive the primitives of stages 1 and 2 (2)
+    int fuse_stage = 2;
     // viscous terms: twelve ghosted flux fields, allocated at the first use
     double *vis_scratch = nullptr;
     size_t vis_scratch_elems = 0;
+    // second set of ghosted primitive fields (d, u, v, w, T, cc) that RK3 stages 1 and 2
+    // write their primitives to, allocated at the first eager RK3 step that can use it
+    double *prim_scratch = nullptr;
+    uint32_t prim_pitch = 0;
 };
 
 KParams make_kparams(const ws_config &cfg);
