@@ -294,6 +294,33 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
         }
     }
     __syncthreads();
+    // The stage's fn and f values of this thread's phase-3 cells go out here, together:
+    // loaded inside the stage arithmetic, each sat behind its own stage-number branch and
+    // was waited on alone.
+    double fnv[STG ? CPT : 1][5], fvv[STG ? CPT : 1][5];
+    if (STG) {
+        const size_t nstride = (size_t)sa.fn.pitch * k.ny * k.nz, fstride = (size_t)sa.f.pitch * k.ny * k.nz;
+#pragma unroll
+        for (int r = 0; r < CPT; ++r) {
+            const int c = t + r * TL::NT;
+            int l, i;
+            if (DIR == 0) { i = c % L; l = c / L; }
+            else          { l = c % W; i = c / W; }
+            if (c >= W * L || l < g.lo_across || l >= g.lim_across || i >= g.lim_line) continue;
+            int cx, cy, cz;
+            if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
+            else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
+            else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
+            const size_t cell = (size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP);
+            const double *np = sa.fn.ptr + (size_t)(cx - LAP) + (size_t)sa.fn.pitch * cell;
+            const double *fp = sa.f.ptr + (size_t)(cx - LAP) + (size_t)sa.f.pitch * cell;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) fnv[r][m] = np[m * nstride];
+            if (sa.krk != 1)
+#pragma unroll
+                for (int m = 0; m < 5; ++m) fvv[r][m] = fp[m * fstride];
+        }
+    }
     // face fluxes reuse the split buffer (its first 5*W*F entries; the M split values, from
     // 5*QS on, stay intact for the non-reflecting fix of phase 3)
     double *nfix_lds = sp + 5 * W * TL::F;        // [W][5], free space below 5*QS
@@ -347,17 +374,7 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                 val = scale * nfix_lds[l * 5 + m];
             }
             if (STG) {
-                const double dtot = duo[r][m] + val;
-                const size_t fo = (size_t)(cx - LAP) +
-                                  (size_t)sa.f.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP)) +
-                                  (size_t)m * sa.f.pitch * k.ny * k.nz;
-                const size_t no = (size_t)(cx - LAP) +
-                                  (size_t)sa.fn.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP)) +
-                                  (size_t)m * sa.fn.pitch * k.ny * k.nz;
-                const double fnv = sa.fn.ptr[no];
-                if (sa.krk == 1) rv[m] = rk3_stage_value<1>(fnv, 0.0, dtot, sdt);
-                else if (sa.krk == 2) rv[m] = rk3_stage_value<2>(fnv, sa.f.ptr[fo], dtot, sdt);
-                else rv[m] = rk3_stage_value<3>(fnv, sa.f.ptr[fo], dtot, sdt);
+                rv[m] = duo[r][m] + val;          // the complete RHS, staged below
             } else if (ACC) {
                 dp[m * vstride] = duo[r][m] + val;
             } else {
@@ -365,6 +382,16 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
             }
         }
         if (STG) {
+            if (sa.krk == 1) {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) rv[m] = rk3_stage_value<1>(fnv[r][m], 0.0, rv[m], sdt);
+            } else if (sa.krk == 2) {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) rv[m] = rk3_stage_value<2>(fnv[r][m], fvv[r][m], rv[m], sdt);
+            } else {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) rv[m] = rk3_stage_value<3>(fnv[r][m], fvv[r][m], rv[m], sdt);
+            }
             // the new f's primitives for the next stage's RHS (and OCFD_bc on the planes)
             if (sa.pd.ptr) {
                 const CellPrims p = cell_primitives(rv, k.cv);
