@@ -312,7 +312,9 @@ int ws_rk3_step(ws_ctx *ctx, ws_state *st, const double *dt_dev, hipStream_t str
 
 /* nsteps RK3 steps.  cfl > 0: dt recomputed on the device at the start of every step
  * with ws_cfl_dt; cfl <= 0: cfg.dt.  use_graph = 1 captures one step in a hipGraph
- * and replays it.  Synchronises the stream before returning. */
+ * and replays it.  Synchronises the stream before returning.  After the first step the
+ * CFL maxima come from the stage kernel that ends the previous step (same dt bits as a
+ * ws_cfl_dt pass; the context keeps one double per 256 cells for them). */
 int ws_rk3_run(ws_ctx *ctx, ws_state *st, int32_t nsteps, double cfl, int32_t use_graph,
                hipStream_t stream);
 

@@ -265,7 +265,9 @@ int prim_set(ws_ctx *c, hipStream_t s, ws_field out[6])
     return 1;
 }
 
-int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
+// cfl_next: the stage kernel that ends the step also leaves the CFL maxima of the new
+// state there (launch_stage's cfl_part).
+int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next)
 {
     std::swap(st->f, st->fn);
     const ws_soa nolap{};
@@ -294,7 +296,7 @@ int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
                 r = c->cfg.precision == WS_PREC_MIXED
                         ? launch_rhs_mixed_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s)
                         : launch_rhs_tiled_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s);
-            if (!r && krk == 3) r = launch_stage(*c, 0, *st, nolap, c->cfg.dt, dt_dev, planes, false, s);
+            if (!r && krk == 3) r = launch_stage(*c, 0, *st, nolap, c->cfg.dt, dt_dev, planes, false, s, cfl_next);
             if (!r && krk < 3) r = launch_ghost_fill(*c, out, 6, s);
             continue;
         }
@@ -304,10 +306,10 @@ int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s)
                 r = c->cfg.precision == WS_PREC_MIXED
                         ? launch_rhs_mixed_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, interior_job(c), sa, s)
                         : launch_rhs_tiled_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, interior_job(c), sa, s);
-            if (!r) r = launch_stage(*c, 0, *st, nolap, c->cfg.dt, dt_dev, planes, sound, s);
+            if (!r) r = launch_stage(*c, 0, *st, nolap, c->cfg.dt, dt_dev, planes, sound, s, krk == 3 ? cfl_next : nullptr);
         } else {
             if (!r) r = du_from_prepared(c, st, s);
-            if (!r) r = launch_stage(*c, krk, *st, nolap, c->cfg.dt, dt_dev, planes, sound, s);
+            if (!r) r = launch_stage(*c, krk, *st, nolap, c->cfg.dt, dt_dev, planes, sound, s, krk == 3 ? cfl_next : nullptr);
         }
         if (!r && sound) r = launch_ghost_fill(*c, g, 6, s);
     }
@@ -389,9 +391,10 @@ int ws_ctx_destroy(ws_ctx *c)
     if (!c) return WS_OK;
     if (c->capture_stream) hipStreamDestroy(c->capture_stream);
     if (c->cfl_partial) hipFree(c->cfl_partial);
-    if (c->vis_scratch || c->prim_scratch) hipDeviceSynchronize();
+    if (c->vis_scratch || c->prim_scratch || c->cfl_stage) hipDeviceSynchronize();
     if (c->vis_scratch) hipFree(c->vis_scratch);
     if (c->prim_scratch) hipFree(c->prim_scratch);
+    if (c->cfl_stage) hipFree(c->cfl_stage);
     delete c;
     return WS_OK;
 }
@@ -660,20 +663,36 @@ int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_
         hipError_t e = hipMalloc(&dt_dev, sizeof(double));
         if (e != hipSuccess) return hip_error(e, "dt allocation");
     }
+    // With a CFL step, the stage kernel that ends each step leaves the maxima of the new
+    // state in c->cfl_stage, and the next step's dt is their reduction (k_cfl_final):
+    // the same maximum as a k_cfl_partial pass over the state, without that pass.
+    double *cfl_next = nullptr;
+    if (dt_dev) {
+        const long n = stage_blocks(*c);
+        if (c->cfl_stage && c->cfl_stage_n != n) { hipStreamSynchronize(s); hipFree(c->cfl_stage); c->cfl_stage = nullptr; }
+        if (!c->cfl_stage && hipMalloc(&c->cfl_stage, sizeof(double) * n) != hipSuccess) c->cfl_stage = nullptr;
+        c->cfl_stage_n = c->cfl_stage ? n : 0;
+        cfl_next = c->cfl_stage;
+    }
+    // one step; the first takes its dt from a pass over the state
+    auto step = [&](ws_state *S, bool first, hipStream_t ss) {
+        int q = WS_OK;
+        if (dt_dev)
+            q = (first || !cfl_next) ? ws_cfl_dt(c, S, cfl, dt_dev, ss)
+                                     : launch_cfl_final(*c, cfl_next, (int)c->cfl_stage_n, cfl, dt_dev, ss);
+        if (!q) q = rk3_step_fused(c, S, dt_dev, ss, cfl_next);
+        return q;
+    };
     int r = WS_OK;
     if (!use_graph) {
-        for (int n = 0; n < nsteps && !r; ++n) {
-            if (dt_dev) r = ws_cfl_dt(c, st, cfl, dt_dev, s);
-            if (!r) r = ws_rk3_step(c, st, dt_dev, s);
-        }
+        for (int n = 0; n < nsteps && !r; ++n) r = step(st, n == 0, s);
     } else {
         // Step 0 runs eagerly (it performs any lazy allocation); later steps replay one
         // captured graph per parity of the f/fn swap, since a step ends with f and fn exchanged.
         hipGraphExec_t exec[2] = {nullptr, nullptr};
         for (int n = 0; n < nsteps && !r; ++n) {
             if (n == 0) {
-                if (dt_dev) r = ws_cfl_dt(c, st, cfl, dt_dev, s);
-                if (!r) r = ws_rk3_step(c, st, dt_dev, s);
+                r = step(st, true, s);
                 continue;
             }
             const int p = n & 1;
@@ -683,8 +702,7 @@ int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_
                 hipStream_t cs = c->capture_stream;
                 hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) { r = hip_error(e, "begin capture"); break; }
-                if (dt_dev) r = ws_cfl_dt(c, &probe, cfl, dt_dev, cs);
-                if (!r) r = ws_rk3_step(c, &probe, dt_dev, cs);
+                r = step(&probe, false, cs);
                 e = hipStreamEndCapture(cs, &g);
                 if (r) { if (g) hipGraphDestroy(g); break; }
                 if (e != hipSuccess) { r = hip_error(e, "end capture"); break; }

@@ -164,16 +164,15 @@ __global__ void __launch_bounds__(NT) k_time_advance(KParams k, ws_soa f, ws_soa
 // Same expressions with contraction off as the separate kernels, so the state is
 // bit-identical; f is read once and the primitives are not read back.  KRK = 0: the last
 // RHS launch already wrote the stage's f; this reads it, writes the primitives, and
-// writes f back only on the boundary planes it re-derives.
-template <int KRK, bool PLANES, bool SOUND>
-__global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl, double dt_val,
-                                              const double *dt_dev, ws_field d, ws_field u, ws_field v, ws_field w,
-                                              ws_field T, ws_field P, ws_field cc)
+// writes f back only on the boundary planes it re-derives.  CFLP (the last stage of a
+// step): each work-group also writes the largest max(|u|,|v|,|w|) + c of its cells'
+// new primitives to cfl_part[blockIdx.x], k_cfl_partial's quantity for the next step.
+template <int KRK, bool PLANES, bool SOUND, bool CFLP>
+__device__ __forceinline__ double stage_cell(const KParams &k, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl,
+                                             double dt_val, const double *dt_dev, ws_field d, ws_field u, ws_field v,
+                                             ws_field w, ws_field T, ws_field P, ws_field cc, long i)
 {
 #pragma clang fp contract(off)
-    const long total = (long)k.nx * k.ny * k.nz;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
     const double dt = (KRK != 0 && dt_dev) ? *dt_dev : dt_val;
     const int x = (int)(i % k.nx);
     const int y = (int)((i / k.nx) % k.ny);
@@ -201,12 +200,40 @@ __global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws
     w.ptr[fidx(w.pitch, k.nyg, gx, gy, gz)] = p.w;
     T.ptr[fidx(T.pitch, k.nyg, gx, gy, gz)] = p.T;
     if (P.ptr) P.ptr[fidx(P.pitch, k.nyg, gx, gy, gz)] = p.e / p.d * (k.gamma - 1.0);
-    if (SOUND) cc.ptr[fidx(cc.pitch, k.nyg, gx, gy, gz)] = sqrt(p.T) / k.ama;
+    const double c = sqrt(p.T) / k.ama;
+    if (SOUND) cc.ptr[fidx(cc.pitch, k.nyg, gx, gy, gz)] = c;
     const bool plane = PLANES && (x == 0 || x == k.nx - 1 || y == 0 || y == k.ny - 1 || z == 0 || z == k.nz - 1);
     if (plane) plane_conserved(p, k.cv, r);
     if (KRK != 0 || plane)
 #pragma unroll
         for (int m = 0; m < 5; ++m) f.ptr[sidx(f.pitch, k.ny, k.nz, x, y, z, m)] = r[m];
+    // k_cfl_partial's expression on the primitives just written
+    return CFLP ? fmax(fabs(p.u) + c, fmax(fabs(p.v) + c, fabs(p.w) + c)) : 0.0;
+}
+
+template <int KRK, bool PLANES, bool SOUND, bool CFLP>
+__global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl, double dt_val,
+                                              const double *dt_dev, ws_field d, ws_field u, ws_field v, ws_field w,
+                                              ws_field T, ws_field P, ws_field cc, double *cfl_part)
+{
+    const long total = (long)k.nx * k.ny * k.nz;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double mx = 0.0;
+    if (i < total) mx = stage_cell<KRK, PLANES, SOUND, CFLP>(k, f, fn, du, fl, dt_val, dt_dev, d, u, v, w, T, P, cc, i);
+    if (CFLP) {
+        // fmax is exact and ignores NaN, so the order of the reduction does not matter
+        __shared__ double wmax[NT / 64];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = wmax[0];
+#pragma unroll
+            for (int q = 1; q < NT / 64; ++q) b = fmax(b, wmax[q]);
+            cfl_part[blockIdx.x] = b;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(NT) k_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy,
@@ -252,6 +279,7 @@ __global__ void __launch_bounds__(NT) k_cfl_final(const double *partial, int n, 
 #pragma clang fp contract(off)
     __shared__ double red[NT];
     double mx = 0.0;
+#pragma unroll 8
     for (int i = threadIdx.x; i < n; i += NT) mx = fmax(mx, partial[i]);
     red[threadIdx.x] = mx;
     __syncthreads();
@@ -360,34 +388,43 @@ int launch_time_advance(const Ctx &c, int krk, ws_soa f, ws_soa fn, ws_soa du, w
 }
 
 int launch_stage(const Ctx &c, int krk, const ws_state &st, ws_soa fl, double dt, const double *dt_dev, bool planes,
-                 bool sound, hipStream_t s)
+                 bool sound, hipStream_t s, double *cfl_part)
 {
     const KParams &k = c.kp;
     const dim3 g(blocks_for((long)k.nx * k.ny * k.nz));
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, g, dim3(NT), 0, s, k, st.f, st.fn, st.du, fl, dt, dt_dev, st.d, st.u, st.v, st.w,
-                           st.T, st.P, st.cc);
+                           st.T, st.P, st.cc, cfl_part);
     };
     // krk 0: the primitives (and planes, sound speed) of an f the last RHS launch updated
     if (krk < 0 || krk > 3) return set_error(WS_ERR_INVALID, "krk must be 0, 1, 2 or 3");
+    if (cfl_part) {
+        // the last stage of a step: krk 3, or 0 after the RHS applied stage 3
+        if ((krk != 0 && krk != 3) || sound)
+            return set_error(WS_ERR_INVALID, "the CFL maxima come from the last stage of a step");
+        if (krk == 0) planes ? go(k_stage<0, true, false, true>) : go(k_stage<0, false, false, true>);
+        else planes ? go(k_stage<3, true, false, true>) : go(k_stage<3, false, false, true>);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? WS_OK : hip_error(e, "stage launch");
+    }
     const int sel = (krk == 0 ? 12 : (krk - 1) * 4) + (planes ? 2 : 0) + (sound ? 1 : 0);
     switch (sel) {
-    case 12: go(k_stage<0, false, false>); break;
-    case 13: go(k_stage<0, false, true>); break;
-    case 14: go(k_stage<0, true, false>); break;
-    case 15: go(k_stage<0, true, true>); break;
-    case 0: go(k_stage<1, false, false>); break;
-    case 1: go(k_stage<1, false, true>); break;
-    case 2: go(k_stage<1, true, false>); break;
-    case 3: go(k_stage<1, true, true>); break;
-    case 4: go(k_stage<2, false, false>); break;
-    case 5: go(k_stage<2, false, true>); break;
-    case 6: go(k_stage<2, true, false>); break;
-    case 7: go(k_stage<2, true, true>); break;
-    case 8: go(k_stage<3, false, false>); break;
-    case 9: go(k_stage<3, false, true>); break;
-    case 10: go(k_stage<3, true, false>); break;
-    case 11: go(k_stage<3, true, true>); break;
+    case 12: go(k_stage<0, false, false, false>); break;
+    case 13: go(k_stage<0, false, true, false>); break;
+    case 14: go(k_stage<0, true, false, false>); break;
+    case 15: go(k_stage<0, true, true, false>); break;
+    case 0: go(k_stage<1, false, false, false>); break;
+    case 1: go(k_stage<1, false, true, false>); break;
+    case 2: go(k_stage<1, true, false, false>); break;
+    case 3: go(k_stage<1, true, true, false>); break;
+    case 4: go(k_stage<2, false, false, false>); break;
+    case 5: go(k_stage<2, false, true, false>); break;
+    case 6: go(k_stage<2, true, false, false>); break;
+    case 7: go(k_stage<2, true, true, false>); break;
+    case 8: go(k_stage<3, false, false, false>); break;
+    case 9: go(k_stage<3, false, true, false>); break;
+    case 10: go(k_stage<3, true, false, false>); break;
+    case 11: go(k_stage<3, true, true, false>); break;
     default: return set_error(WS_ERR_INVALID, "krk must be 0, 1, 2 or 3");
     }
     hipError_t e = hipGetLastError();
@@ -416,6 +453,17 @@ int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "cfl launch");
 }
+
+int launch_cfl_final(const Ctx &c, const double *partial, int n, double cfl, double *dt_dev, hipStream_t s)
+{
+    const KParams &k = c.kp;
+    const double hmin = fmin(k.h[0], fmin(k.h[1], k.h[2]));
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(NT), 0, s, partial, n, cfl, hmin, dt_dev);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "cfl launch");
+}
+
+long stage_blocks(const Ctx &c) { return blocks_for((long)c.kp.nx * c.kp.ny * c.kp.nz); }
 
 int launch_min_reduce(const double *v, int n, double *out, hipStream_t s)
 {

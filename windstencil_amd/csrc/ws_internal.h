@@ -79,6 +79,10 @@ struct Ctx {
     // write their primitives to, allocated at the first eager RK3 step that can use it
     double *prim_scratch = nullptr;
     uint32_t prim_pitch = 0;
+    // CFL maxima the last stage of an ws_rk3_run step leaves for the next step's dt
+    // (one per stage-kernel work-group), allocated by the first eager step
+    double *cfl_stage = nullptr;
+    long cfl_stage_n = 0;
 };
 
 KParams make_kparams(const ws_config &cfg);
@@ -150,7 +154,7 @@ int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s);
 int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s);
 // one SSP-RK3 step of st with fused stage kernels (the same arithmetic as du_comput,
 // time_advance, cons_to_pri and bc_planes in turn)
-int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s);
+int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next = nullptr);
 
 // viscous terms added to du[1..4] (k_viscous.hip); allocates the context's scratch once
 int launch_viscous(Ctx &c, ws_field u, ws_field v, ws_field w, ws_field T, const ws_metric_fields *m, ws_soa du,
@@ -172,8 +176,12 @@ int launch_time_advance(const Ctx &c, int krk, ws_soa f, ws_soa fn, ws_soa du, w
                         double dt, const double *dt_dev, hipStream_t s);
 // one fused RK3 stage on st: f from (f, fn, du), the primitives (and P) from the new f, with
 // planes the boundary-plane re-derivation of f, with sound the interior sound speed in cc
+// cfl_part: the last stage of a step also writes one CFL maximum per work-group there
+// (stage_blocks entries) for launch_cfl_final
 int launch_stage(const Ctx &c, int krk, const ws_state &st, ws_soa f_lap, double dt, const double *dt_dev,
-                 bool planes, bool sound, hipStream_t s);
+                 bool planes, bool sound, hipStream_t s, double *cfl_part = nullptr);
+long stage_blocks(const Ctx &c);
+int launch_cfl_final(const Ctx &c, const double *partial, int n, double cfl, double *dt_dev, hipStream_t s);
 int launch_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy, uint32_t sz,
                 hipStream_t s);
 int launch_cfl(Ctx &c, const ws_state &st, double cfl, double *dt_dev, hipStream_t s);
