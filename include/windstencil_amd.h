@@ -400,9 +400,9 @@ int ws_batch_set_kernel(ws_batch *batch, int32_t kernel);
 int ws_batch_set_tile_shape(ws_batch *batch, int32_t shape);
 int ws_batch_get_tile_shape(const ws_batch *batch, int32_t *shape);
 int ws_batch_set_schedule(ws_batch *batch, int32_t schedule);
-/* As ws_ctx_set_stage_fusion, for the batch's RK3 steps: nonzero lets the batched z launch
- * apply the stage (grid schedule, no viscous terms); the primitives of a batch are always
- * derived by the stage kernel. */
+/* As ws_ctx_set_stage_fusion, for the batch's RK3 steps (grid schedule, no viscous terms):
+ * 1 lets the batched z launch apply the stage, 2 (default) also derive the primitives of
+ * stages 1 and 2 there (each block's context then holds a second primitive set). */
 int ws_batch_set_stage_fusion(ws_batch *batch, int32_t on);
 int ws_batch_get_schedule(const ws_batch *batch, int32_t *schedule);
 /* RHS of every block from ghost-filled d,u,v,w and cc (overwrites du). */

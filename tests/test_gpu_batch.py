@@ -128,6 +128,41 @@ def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision, schedule):
         assert rel_l2(st.interior("f").cpu().numpy(), ref.interior("f").cpu().numpy()) <= 1e-12, e
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+def test_batch_stage_fusion_levels(ws, precision, graph):
+    """Batched RK3 steps (fixed dt, boundary planes on) with the z launch applying each
+    stage (level 1) and also deriving the primitives of stages 1 and 2 into every block's
+    second primitive set (level 2, the default).  Levels 2 and 1 run the same kernels, so
+    f and every ghosted primitive field agree bit for bit.  Against the separate stage
+    kernels (level 0) the stage-applying z launch is another instantiation of the tile
+    code, which the compiler may contract differently (fused multiply-adds in the WENO and
+    split arithmetic): the states agree to rounding, 1e-14 relative L2 after 3 steps."""
+    extents = _extents(19, 5, 6, 30)
+    runs = []
+    for level in (0, 1, 2):
+        b, _ = _batch(extents, 700, precision=precision, bc_planes=1, dt=2e-4)
+        W.check(W.lib().ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
+        W.check(W.lib().ws_batch_set_stage_fusion(b.handle, level))
+        for e, st in zip(extents, b.states):
+            _to_cons(make_ctx(*e), st)
+        log(f"batch fusion {level} prec {precision} graph {graph}")
+        W.check(W.lib().ws_batch_rk3_run(b.handle, 3, -1.0, graph, W.stream_handle()))
+        b.sync_states()
+        _sync()
+        runs.append([{k: st.t[k].cpu().numpy() for k in ("f", "d", "u", "v", "w", "T", "cc")}
+                     for st in b.states])
+        b.close()
+    for i, (a, c) in enumerate(zip(runs[1], runs[2])):
+        for k in a:
+            assert np.array_equal(a[k], c[k]), (i, k)
+    for i, (a, c) in enumerate(zip(runs[0], runs[1])):
+        assert np.isfinite(a["f"]).all()
+        err = rel_l2(c["f"], a["f"])
+        log(f"  block {i}: level 1 vs 0 rel L2 {err:.2e}")
+        assert err <= 1e-14, (i, err)
+
+
 @pytest.mark.parametrize("schedule", [W.WS_SCHED_GRID, W.WS_SCHED_PREFETCH])
 @pytest.mark.parametrize("nsteps", [2, 3, 5])
 def test_batch_graph_replay_equals_eager(ws, nsteps, schedule):

@@ -29,6 +29,10 @@ struct ws_batch {
     std::vector<ws_ctx *> ctx;
     std::vector<ws_state> st;
     BatchBlock *d_blocks = nullptr;
+    // the same table with the primitive inputs taken from each block's second primitive set
+    // and the outputs (od..occ) the block's own fields; d_blocks' outputs are the second set
+    BatchBlock *d_blocks2 = nullptr;
+    std::vector<BatchBlock> host_blocks;
     std::vector<double *> fas, fbs;   // each block's f and fn as created (the table's fa, fb)
     double *fb0 = nullptr;
     int4 *d_tiles[WS_TILE_COUNT][3] = {};
@@ -78,7 +82,7 @@ BatchTiles tiles_of(const ws_batch *b)
     return t;
 }
 
-int batch_rhs(ws_batch *b, hipStream_t s, const BatchStage *stage = nullptr)
+int batch_rhs(ws_batch *b, hipStream_t s, const BatchStage *stage = nullptr, const BatchBlock *table = nullptr)
 {
     if (b->kernel == WS_KERNEL_ANCHOR) {
         // the reference-order kernels have no batched form: one RHS per block
@@ -91,7 +95,39 @@ int batch_rhs(ws_batch *b, hipStream_t s, const BatchStage *stage = nullptr)
     const KParams &k = b->ctx[0]->kp;   // physics constants and metrics are the batch's
     BatchTiles t = tiles_of(b);
     t.stage = stage;
+    if (table) t.blocks = table;
     return b->precision == WS_PREC_MIXED ? launch_rhs_batch_mixed(k, t, s) : launch_rhs_batch_tiled(k, t, s);
+}
+
+// Both block tables for RK3 steps whose stage-applying launch also derives the primitives:
+// the blocks' second primitive sets (their contexts allocate them) become the outputs of
+// d_blocks and the inputs of d_blocks2.  Built once, outside a stream capture.
+bool prim_tables(ws_batch *b, hipStream_t s)
+{
+    if (b->d_blocks2) return true;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+    const size_t n = b->host_blocks.size();
+    std::vector<BatchBlock> a = b->host_blocks, c = b->host_blocks;
+    for (size_t i = 0; i < n; ++i) {
+        ws_field sec[6];
+        if (!prim_set(b->ctx[i], s, sec)) return false;
+        const ws_state &st = b->st[i];
+        a[i].od = sec[0]; a[i].ou = sec[1]; a[i].ov = sec[2]; a[i].ow = sec[3]; a[i].oT = sec[4]; a[i].occ = sec[5];
+        c[i].d = sec[0]; c[i].u = sec[1]; c[i].v = sec[2]; c[i].w = sec[3]; c[i].cc = sec[5];
+        c[i].od = st.d; c[i].ou = st.u; c[i].ov = st.v; c[i].ow = st.w; c[i].oT = st.T; c[i].occ = st.cc;
+    }
+    BatchBlock *d2 = nullptr;
+    if (hipMalloc(&d2, sizeof(BatchBlock) * n) != hipSuccess) return false;
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        hipMemcpy(d2, c.data(), sizeof(BatchBlock) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_blocks, a.data(), sizeof(BatchBlock) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d2);
+        return false;
+    }
+    b->host_blocks = a;
+    b->d_blocks2 = d2;
+    return true;
 }
 
 }  // namespace
@@ -140,6 +176,7 @@ int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *exten
     hipError_t e = hipMalloc(&b->d_blocks, sizeof(BatchBlock) * nblocks);
     if (e != hipSuccess) return fail(b, set_error(WS_ERR_NOMEM, "batch block table"));
     e = hipMemcpy(b->d_blocks, host.data(), sizeof(BatchBlock) * nblocks, hipMemcpyHostToDevice);
+    b->host_blocks = host;
     if (e != hipSuccess) return fail(b, hip_error(e, "batch block table upload"));
     for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
         for (int dir = 0; dir < 3; ++dir) {
@@ -169,6 +206,7 @@ int ws_batch_destroy(ws_batch *b)
     hipDeviceSynchronize();
     for (ws_ctx *c : b->ctx) ws_ctx_destroy(c);
     if (b->d_blocks) hipFree(b->d_blocks);
+    if (b->d_blocks2) hipFree(b->d_blocks2);
     for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
         for (int dir = 0; dir < 3; ++dir)
             if (b->d_tiles[shape][dir]) hipFree(b->d_tiles[shape][dir]);
@@ -217,7 +255,8 @@ int ws_batch_set_tile_shape(ws_batch *b, int32_t shape)
 int ws_batch_set_stage_fusion(ws_batch *b, int32_t on)
 {
     if (!b) return set_error(WS_ERR_INVALID, "null batch");
-    for (ws_ctx *c : b->ctx) c->fuse_stage = on ? 1 : 0;
+    if (on < 0 || on > 2) return set_error(WS_ERR_INVALID, "stage fusion must be 0, 1 or 2");
+    for (ws_ctx *c : b->ctx) c->fuse_stage = on;
     return WS_OK;
 }
 
@@ -291,11 +330,28 @@ int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
     const int swapped = b->st[0].f.ptr == b->fb0 ? 1 : 0;
     for (size_t i = 0; i < b->st.size() && fuse; ++i)
         fuse = b->st[i].f.ptr == (swapped ? b->fbs[i] : b->fas[i]);
+    // Fusion level 2 (as for a single box): the z launch of stages 1 and 2 also writes the
+    // primitives, stage 1 from the blocks' own fields into their second sets (d_blocks),
+    // stage 2 back (d_blocks2); a ghost fill of the six written fields follows.
+    const bool prims = fuse && b->ctx[0]->fuse_stage > 1 && prim_tables(b, s);
     // the blocks are periodic (ws_batch_create); stages 1 and 2 leave the next stage's
     // inputs prepared (fused stage kernel writing the sound speed, then one ghost fill)
     for (int krk = 1; krk <= 3 && !r; ++krk) {
         for (size_t i = 0; i < b->ctx.size() && !r && krk == 1; ++i) r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
-        const BatchStage bs{dt_dev, b->base.dt, krk, swapped};
+        BatchStage bs{dt_dev, b->base.dt, krk, swapped};
+        if (prims && krk < 3) {
+            bs.prims = 1;
+            bs.planes = b->base.bc_planes != 0;
+            if (!r) r = batch_rhs(b, s, &bs, krk == 2 ? b->d_blocks2 : b->d_blocks);
+            for (size_t i = 0; i < b->ctx.size() && !r; ++i) {
+                const BatchBlock &h = b->host_blocks[i];
+                const ws_state &st = b->st[i];
+                const ws_field g1[6] = {h.od, h.ou, h.ov, h.ow, h.oT, h.occ};
+                const ws_field g2[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
+                r = launch_ghost_fill(*b->ctx[i], krk == 1 ? g1 : g2, 6, s);
+            }
+            continue;
+        }
         if (!r) r = batch_rhs(b, s, fuse ? &bs : nullptr);
         for (size_t i = 0; i < b->ctx.size() && !r && b->base.viscous; ++i) {
             const ws_state &st = b->st[i];

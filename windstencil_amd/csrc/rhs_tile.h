@@ -374,9 +374,9 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                 val = scale * nfix_lds[l * 5 + m];
             }
             if (STG) {
-                rv[m] = duo[r][m] + val;          // the complete RHS, staged below
+                rv[m] = acc_sum(duo[r][m], val);   // the complete RHS, staged below
             } else if (ACC) {
-                dp[m * vstride] = duo[r][m] + val;
+                dp[m * vstride] = acc_sum(duo[r][m], val);
             } else {
                 dp[m * vstride] = val;
             }
@@ -557,7 +557,13 @@ k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int nti
     tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
     const ws_metric_fields nomf{};
     StageArgs sa{};
-    if (STG) sa = StageArgs{bs.swapped ? b.fb : b.fa, bs.swapped ? b.fa : b.fb, bs.dt_dev, bs.dt, bs.krk};
+    if (STG) {
+        sa = StageArgs{bs.swapped ? b.fb : b.fa, bs.swapped ? b.fa : b.fb, bs.dt_dev, bs.dt, bs.krk};
+        if (bs.prims) {
+            sa.planes = bs.planes;
+            sa.pd = b.od; sa.pu = b.ou; sa.pv = b.ov; sa.pw = b.ow; sa.pT = b.oT; sa.pcc = b.occ;
+        }
+    }
     tile_compute<DIR, L, W, K, false, ACC, MIX, false, STG>(kb, nomf, b.du, g, ld, sp, sa, [] {});
 }
 

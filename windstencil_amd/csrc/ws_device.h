@@ -133,6 +133,15 @@ __device__ __forceinline__ double rk3_stage_value(double fnv, double fv, double 
     return t1 * fnv + t2 * (fv + dt * duv);
 }
 
+// du + this direction's flux difference, without contraction: a launch that adds to du and
+// one that applies the RK3 stage to the sum then hold the same bits whatever the compiler
+// fuses in either.
+__device__ __forceinline__ double acc_sum(double duv, double val)
+{
+#pragma clang fp contract(off)
+    return duv + val;
+}
+
 // Primitives of one cell's conserved values r (get_duvwT, src/kernels/commen_kernel.cpp:78-110)
 // with contraction off: e is the internal energy per volume, T = e / (d Cv).  Shared by the
 // stage kernel and the RHS launch that derives the primitives itself.

@@ -92,18 +92,24 @@ struct BatchBlock {
     ws_field d, u, v, w, cc;
     ws_soa du;
     ws_soa fa, fb;       // the block's f and fn as created (RK3 steps exchange them)
+    // where a stage-applying launch with BatchStage::prims writes the new primitives
+    // (d, u, v, w, T, cc): the other primitive set of the block
+    ws_field od, ou, ov, ow, oT, occ;
     int n[3];            // interior extents
     int pad;
     double rh[3];        // 1/h per direction
 };
 
 // The RK3 stage the last batched RHS launch applies: every block's f is fb and fn fa when
-// `swapped`, else f = fa and fn = fb.
+// `swapped`, else f = fa and fn = fb.  prims: the launch also writes the new primitives to
+// each block's od..occ (and, with planes, re-derives f on the boundary planes).
 struct BatchStage {
     const double *dt_dev;
     double dt;
     int krk;
     int swapped;
+    int prims = 0;
+    int planes = 0;
 };
 
 // Device tile lists of a batch for one tile shape: entry (block, bx, by, bz) per tile,
@@ -155,6 +161,9 @@ int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s);
 // one SSP-RK3 step of st with fused stage kernels (the same arithmetic as du_comput,
 // time_advance, cons_to_pri and bc_planes in turn)
 int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next = nullptr);
+// the context's second set of ghosted primitive fields (d, u, v, w, T, cc), allocated at
+// the first call outside a stream capture; 0 if it is not available
+int prim_set(ws_ctx *c, hipStream_t s, ws_field out[6]);
 
 // viscous terms added to du[1..4] (k_viscous.hip); allocates the context's scratch once
 int launch_viscous(Ctx &c, ws_field u, ws_field v, ws_field w, ws_field T, const ws_metric_fields *m, ws_soa du,
