@@ -163,6 +163,38 @@ def test_batch_stage_fusion_levels(ws, precision, graph):
         assert err <= 1e-14, (i, err)
 
 
+@pytest.mark.parametrize("fusion", [0, 2])
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_run_cfl_from_stage_maxima_bitwise(ws, graph, fusion):
+    """ws_batch_rk3_run takes each step's dt (after the first) from the CFL maxima the
+    blocks' last stage kernels left; steps driven one by one with ws_batch_cfl_dt passes
+    over the state give the same states bit for bit."""
+    import torch
+    extents = _extents(23, 4, 6, 30)
+    s = W.stream_handle()
+    runs = []
+    for manual in (False, True):
+        b, _ = _batch(extents, 800, bc_planes=1)
+        W.check(W.lib().ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
+        W.check(W.lib().ws_batch_set_stage_fusion(b.handle, fusion))
+        for e, st in zip(extents, b.states):
+            _to_cons(make_ctx(*e), st)
+        if manual:
+            dt = torch.zeros(1, dtype=torch.float64, device="cuda")
+            for _ in range(4):
+                W.check(W.lib().ws_batch_cfl_dt(b.handle, 0.4, ctypes.c_void_p(dt.data_ptr()), s))
+                W.check(W.lib().ws_batch_rk3_step(b.handle, ctypes.c_void_p(dt.data_ptr()), s))
+        else:
+            W.check(W.lib().ws_batch_rk3_run(b.handle, 4, 0.4, graph, s))
+        b.sync_states()
+        _sync()
+        runs.append([st.interior("f").cpu().numpy() for st in b.states])
+        b.close()
+    for i, (a, c) in enumerate(zip(*runs)):
+        assert np.isfinite(a).all(), i
+        assert np.array_equal(a, c), i
+
+
 @pytest.mark.parametrize("schedule", [W.WS_SCHED_GRID, W.WS_SCHED_PREFETCH])
 @pytest.mark.parametrize("nsteps", [2, 3, 5])
 def test_batch_graph_replay_equals_eager(ws, nsteps, schedule):

@@ -315,7 +315,9 @@ int ws_batch_cfl_dt(ws_batch *b, double cfl, double *dt_dev, hipStream_t s)
     return launch_min_reduce(b->d_dts, (int)b->ctx.size(), dt_dev, s);
 }
 
-int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
+// One batched RK3 step; cfl_next: the per-block stage kernels that end it also leave the
+// blocks' CFL maxima for the next step's dt (each block's cfl_stage_buffer).
+static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl_next)
 {
     if (!b) return set_error(WS_ERR_INVALID, "null batch");
     for (ws_state &st : b->st) std::swap(st.f, st.fn);
@@ -360,7 +362,8 @@ int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
         for (size_t i = 0; i < b->ctx.size() && !r; ++i) {
             ws_ctx *c = b->ctx[i];
             const ws_state &st = b->st[i];
-            r = launch_stage(*c, fuse ? 0 : krk, st, nolap, c->cfg.dt, dt_dev, c->cfg.bc_planes != 0, krk < 3, s);
+            r = launch_stage(*c, fuse ? 0 : krk, st, nolap, c->cfg.dt, dt_dev, c->cfg.bc_planes != 0, krk < 3, s,
+                             cfl_next && krk == 3 ? c->cfl_stage : nullptr);
             if (!r && krk < 3) {
                 const ws_field g[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
                 r = launch_ghost_fill(*c, g, 6, s);
@@ -370,15 +373,34 @@ int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
     return r;
 }
 
+int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
+{
+    return batch_step(b, dt_dev, s, false);
+}
+
 int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t s)
 {
     if (!b || nsteps < 0) return set_error(WS_ERR_INVALID, "bad RK3 request");
     double *dt_dev = cfl > 0.0 ? b->d_dt : nullptr;
+    // After the first step each block's CFL maxima come from its stage kernel that ended
+    // the previous step (as in ws_rk3_run): per block k_cfl_final over those, then the
+    // minimum over the blocks; the same dt bits as ws_batch_cfl_dt.
+    bool staged = dt_dev != nullptr;
+    for (ws_ctx *c : b->ctx) staged = staged && cfl_stage_buffer(c);
+    auto cfl_dt = [&](bool first, hipStream_t ss) {
+        if (first || !staged) return ws_batch_cfl_dt(b, cfl, dt_dev, ss);
+        for (size_t i = 0; i < b->ctx.size(); ++i) {
+            ws_ctx *c = b->ctx[i];
+            const int q = launch_cfl_final(*c, c->cfl_stage, (int)c->cfl_stage_n, cfl, b->d_dts + i, ss);
+            if (q) return q;
+        }
+        return launch_min_reduce(b->d_dts, (int)b->ctx.size(), dt_dev, ss);
+    };
     int r = WS_OK;
     if (!use_graph) {
         for (int n = 0; n < nsteps && !r; ++n) {
-            if (dt_dev) r = ws_batch_cfl_dt(b, cfl, dt_dev, s);
-            if (!r) r = ws_batch_rk3_step(b, dt_dev, s);
+            if (dt_dev) r = cfl_dt(n == 0, s);
+            if (!r) r = batch_step(b, dt_dev, s, staged);
         }
     } else {
         // one captured graph per parity of the f/fn swap (a step ends with them exchanged);
@@ -386,8 +408,8 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
         hipGraphExec_t exec[2] = {nullptr, nullptr};
         for (int n = 0; n < nsteps && !r; ++n) {
             if (n == 0) {
-                if (dt_dev) r = ws_batch_cfl_dt(b, cfl, dt_dev, s);
-                if (!r) r = ws_batch_rk3_step(b, dt_dev, s);
+                if (dt_dev) r = cfl_dt(true, s);
+                if (!r) r = batch_step(b, dt_dev, s, staged);
                 continue;
             }
             const int p = n & 1;
@@ -396,8 +418,8 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) { r = hip_error(e, "begin capture"); break; }
-                if (dt_dev) r = ws_batch_cfl_dt(b, cfl, dt_dev, b->capture_stream);
-                if (!r) r = ws_batch_rk3_step(b, dt_dev, b->capture_stream);
+                if (dt_dev) r = cfl_dt(false, b->capture_stream);
+                if (!r) r = batch_step(b, dt_dev, b->capture_stream, staged);
                 e = hipStreamEndCapture(b->capture_stream, &g);
                 b->st = saved;   // capture recorded the step; the swap happens at replay
                 if (r) { if (g) hipGraphDestroy(g); break; }

@@ -267,6 +267,16 @@ int prim_set(ws_ctx *c, hipStream_t s, ws_field out[6])
 
 // cfl_next: the stage kernel that ends the step also leaves the CFL maxima of the new
 // state there (launch_stage's cfl_part).
+double *cfl_stage_buffer(ws_ctx *c)
+{
+    if (!c->cfl_stage) {
+        const long n = stage_blocks(*c);
+        if (hipMalloc(&c->cfl_stage, sizeof(double) * n) != hipSuccess) c->cfl_stage = nullptr;
+        c->cfl_stage_n = c->cfl_stage ? n : 0;
+    }
+    return c->cfl_stage;
+}
+
 int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next)
 {
     std::swap(st->f, st->fn);
@@ -666,14 +676,7 @@ int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_
     // With a CFL step, the stage kernel that ends each step leaves the maxima of the new
     // state in c->cfl_stage, and the next step's dt is their reduction (k_cfl_final):
     // the same maximum as a k_cfl_partial pass over the state, without that pass.
-    double *cfl_next = nullptr;
-    if (dt_dev) {
-        const long n = stage_blocks(*c);
-        if (c->cfl_stage && c->cfl_stage_n != n) { hipStreamSynchronize(s); hipFree(c->cfl_stage); c->cfl_stage = nullptr; }
-        if (!c->cfl_stage && hipMalloc(&c->cfl_stage, sizeof(double) * n) != hipSuccess) c->cfl_stage = nullptr;
-        c->cfl_stage_n = c->cfl_stage ? n : 0;
-        cfl_next = c->cfl_stage;
-    }
+    double *cfl_next = dt_dev ? cfl_stage_buffer(c) : nullptr;
     // one step; the first takes its dt from a pass over the state
     auto step = [&](ws_state *S, bool first, hipStream_t ss) {
         int q = WS_OK;

@@ -164,6 +164,10 @@ int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s,
 // the context's second set of ghosted primitive fields (d, u, v, w, T, cc), allocated at
 // the first call outside a stream capture; 0 if it is not available
 int prim_set(ws_ctx *c, hipStream_t s, ws_field out[6]);
+// the context's buffer for the CFL maxima of the stage kernel that ends a step
+// (stage_blocks entries), allocated at the first call (outside a stream capture); null if
+// it cannot be allocated
+double *cfl_stage_buffer(ws_ctx *c);
 
 // viscous terms added to du[1..4] (k_viscous.hip); allocates the context's scratch once
 int launch_viscous(Ctx &c, ws_field u, ws_field v, ws_field w, ws_field T, const ws_metric_fields *m, ws_soa du,
