@@ -193,6 +193,34 @@ def test_stage_applied_by_the_last_rhs_launch_equals_the_stage_kernel(ws, precis
         assert np.array_equal(sa.t[k].cpu().numpy(), sb.t[k].cpu().numpy()), k
 
 
+@pytest.mark.parametrize("fusion", [0, 2])
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+def test_run_cfl_from_stage_maxima_bitwise(ws, precision, fusion):
+    """ws_rk3_run takes each step's dt (after the first) from the CFL maxima its last stage
+    kernel left; steps driven one by one with a ws_cfl_dt pass over the state give the
+    same state bit for bit (tiled kernels, graph replay)."""
+    import torch
+    prims = random_state(41, 22, 17, 21, "vortex")
+    a, _ = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=True, cfl=0.45, precision=precision, fusion=fusion)
+    nz, ny, nx = prims[0].shape
+    L = W.lib()
+    s = W.stream_handle()
+    ctx = make_ctx(nx, ny, nz, kernel=W.WS_KERNEL_TILED, precision=precision)
+    ctx.set_stage_fusion(fusion)
+    st = upload_state(prims)
+    W.check(L.ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                             W.job((0, 0, 0), (nx, ny, nz)), s))
+    dt = torch.zeros(1, dtype=torch.float64, device="cuda")
+    for _ in range(4):
+        W.check(L.ws_cfl_dt(ctx.handle, ctypes.byref(st.ws), 0.45, ctypes.c_void_p(dt.data_ptr()), s))
+        W.check(L.ws_rk3_step(ctx.handle, ctypes.byref(st.ws), ctypes.c_void_p(dt.data_ptr()), s))
+    st.sync_from_ws()
+    _sync()
+    b = st.interior("f").cpu().numpy()
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("nsteps", [2, 3, 4, 5])
 def test_graph_replay_equals_eager(ws, nsteps):
     """Every replay of the captured step (steps 1..n-1) against the eager path."""
