@@ -290,6 +290,19 @@ int ws_probe_valu(int32_t precision, int32_t op, double *tera_ops, hipStream_t s
 int ws_ana_state(ws_ctx *ctx, const ws_state *st, const ws_metric_fields *metrics, double dt,
                  ws_ana_report *report, hipStream_t stream);
 
+/* ws_ana_state without any device allocation: the partial records live in a fixed device
+ * array of the library (513 records, about 37 KB per GPU, held until the process ends), so
+ * the call makes no hipMalloc or hipFree (ws_ana_state allocates and frees its records, and
+ * the free waits for the device).  The array's size does not depend on the box (the pass is
+ * grid-strided), so no box is too large for it; boxes beyond 2^40 ghosted cells are refused
+ * as by ws_ana_state.  Every call on one GPU uses that GPU's array: calls that overlap on
+ * one GPU (two host threads, or two streams) race and must be serialised by the caller.
+ * Same arguments, report and synchronisation as ws_ana_state.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight solver loops that check
+ * the state every step, code under stream capture, harnesses that forbid hipMalloc). */
+int ws_ana_state_noalloc(ws_ctx *ctx, const ws_state *st, const ws_metric_fields *metrics, double dt,
+                         ws_ana_report *report, hipStream_t stream);
+
 /* Viscous terms of the RHS added to du components 1..4 (the reference's du_vis_Jacobian3d,
  * src/solver/OCFD_Stream.cpp:229-276, with du_viscous_Jacobian3d_{init,[xyz]_init,[xyz]_final},
  * src/solver/OCFD_NS_Jacobian3d.cpp:455-789): Sutherland viscosity (get_Amu,

@@ -127,6 +127,8 @@ _SIGS = {
     "ws_probe_valu": (ctypes.c_int, [_i32, _i32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "ws_ana_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_double,
                                     ctypes.POINTER(WsAnaReport), ctypes.c_void_p]),
+    "ws_ana_state_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p,
+                                            ctypes.c_double, ctypes.POINTER(WsAnaReport), ctypes.c_void_p]),
     "ws_rk3_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.c_void_p, ctypes.c_void_p]),
     "ws_rk3_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), _i32, ctypes.c_double, _i32,
                                   ctypes.c_void_p]),

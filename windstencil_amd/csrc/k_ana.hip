@@ -9,6 +9,8 @@
 // records.  No atomics: the report is reproducible.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "ws_device.h"
 
 namespace ws {
@@ -94,31 +96,50 @@ __global__ void __launch_bounds__(AT) k_ana_final(const AnaRec *partial, int n, 
     if (threadIdx.x == 0) *out = r;
 }
 
-}  // namespace
+// Device scratch of ws_ana_state_noalloc: the partial records of k_ana_partial and the
+// final record.  Fixed size (ABLOCKS + 1 records of 72 bytes, about 37 KB) whatever the
+// box, since the pass is grid-strided over ABLOCKS work-groups; each device holds its own
+// copy until the process ends.  In host code `scratch` names the host-side shadow, not the
+// device array: kernels get the address hipGetSymbolAddress returns for the current
+// device (ana_scratch_ptr), never `scratch` itself.
+constexpr int ANA_SCRATCH_RECS = ABLOCKS + 1;
+static __device__ __align__(256) AnaRec scratch[ANA_SCRATCH_RECS];
 
-}  // namespace ws
+constexpr int ANA_MAX_DEVICES = 64;
+std::mutex g_ana_mutex;
+AnaRec *g_ana_scratch[ANA_MAX_DEVICES] = {};
 
-using namespace ws;
-
-extern "C" int ws_ana_state(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics, double dt,
-                            ws_ana_report *rep, hipStream_t s)
+// The device address of `scratch` on the current device, looked up once per device.
+int ana_scratch_ptr(AnaRec **out)
 {
-    if (!c || !st || !rep) return set_error(WS_ERR_INVALID, "null argument");
-    if (!st->d.ptr || !st->u.ptr || !st->v.ptr || !st->w.ptr || !st->T.ptr)
-        return set_error(WS_ERR_INVALID, "state primitives not allocated");
-    if (metrics && !metrics->ajac.ptr) return set_error(WS_ERR_INVALID, "metric fields without Ajac");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_error(e, "hipGetDevice");
+    if (dev < 0 || dev >= ANA_MAX_DEVICES) return set_error(WS_ERR_UNSUPPORTED, "device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_ana_mutex);
+    if (!g_ana_scratch[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return hip_error(e, "analysis scratch: hipGetSymbolAddress");
+        g_ana_scratch[dev] = static_cast<AnaRec *>(ptr);
+    }
+    *out = g_ana_scratch[dev];
+    return WS_OK;
+}
+
+// The pass and its report; buf holds ABLOCKS + 1 records of device memory.  Every record
+// is written before it is read, so buf needs no clearing between calls.
+int ana_run(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics, double dt, ws_ana_report *rep,
+            hipStream_t s, AnaRec *buf)
+{
     const KParams &k = c->kp;
-    AnaRec *buf = nullptr;
-    hipError_t e = hipMalloc(&buf, sizeof(AnaRec) * (ABLOCKS + 1));
-    if (e != hipSuccess) return hip_error(e, "analysis scratch");
     const ws_field aj = metrics ? metrics->ajac : ws_field{};
     hipLaunchKernelGGL(k_ana_partial, dim3(ABLOCKS), dim3(AT), 0, s, k, st->d, st->u, st->v, st->w, st->T, aj, buf);
     hipLaunchKernelGGL(k_ana_final, dim3(1), dim3(AT), 0, s, (const AnaRec *)buf, ABLOCKS, buf + ABLOCKS);
     AnaRec r{};
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&r, buf + ABLOCKS, sizeof(AnaRec), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(buf);
     if (e != hipSuccess) return hip_error(e, "state analysis");
     const long long nxg = k.nx + 2 * LAP, nyg = k.nyg;
     int64_t *cnt[4] = {&rep->nan_T, &rep->neg_T, &rep->neg_d, &rep->neg_jac};
@@ -138,4 +159,45 @@ extern "C" int ws_ana_state(ws_ctx *c, const ws_state *st, const ws_metric_field
     rep->cfl = dt > 0.0 ? dt * r.smax / hmin : 0.0;
     rep->fatal = (rep->nan_T > 0 || rep->neg_T > 10) ? 1 : 0;
     return WS_OK;
+}
+
+int ana_check(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics, ws_ana_report *rep)
+{
+    if (!c || !st || !rep) return set_error(WS_ERR_INVALID, "null argument");
+    if (!st->d.ptr || !st->u.ptr || !st->v.ptr || !st->w.ptr || !st->T.ptr)
+        return set_error(WS_ERR_INVALID, "state primitives not allocated");
+    if (metrics && !metrics->ajac.ptr) return set_error(WS_ERR_INVALID, "metric fields without Ajac");
+    // the ghosted linear index of the pass and of the first offending cells is 64-bit
+    const long long cells = (long long)(c->kp.nx + 2 * LAP) * c->kp.nyg * c->kp.nzg;
+    if (cells <= 0 || cells > (1LL << 40)) return set_error(WS_ERR_INVALID, "ghosted box too large for the analysis pass");
+    return WS_OK;
+}
+
+}  // namespace
+
+}  // namespace ws
+
+using namespace ws;
+
+extern "C" int ws_ana_state(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics, double dt,
+                            ws_ana_report *rep, hipStream_t s)
+{
+    int r = ana_check(c, st, metrics, rep);
+    if (r) return r;
+    AnaRec *buf = nullptr;
+    hipError_t e = hipMalloc(&buf, sizeof(AnaRec) * ANA_SCRATCH_RECS);
+    if (e != hipSuccess) return hip_error(e, "analysis scratch");
+    r = ana_run(c, st, metrics, dt, rep, s, buf);
+    (void)hipFree(buf);
+    return r;
+}
+
+extern "C" int ws_ana_state_noalloc(ws_ctx *c, const ws_state *st, const ws_metric_fields *metrics, double dt,
+                                    ws_ana_report *rep, hipStream_t s)
+{
+    int r = ana_check(c, st, metrics, rep);
+    if (r) return r;
+    AnaRec *buf = nullptr;
+    if ((r = ana_scratch_ptr(&buf))) return r;
+    return ana_run(c, st, metrics, dt, rep, s, buf);
 }
