@@ -31,9 +31,9 @@ PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 ALGO_BYTES_PER_CELL = 80     # 5 FP64 state in + 5 FP64 RHS out
-# viscous kernels: u, v, w, T in (32 B) and twelve flux fields out (96 B); the twelve
-# fluxes in again (96 B) and du[1..4] read and written (64 B)
-VISCOUS_BYTES_PER_CELL = 288
+# the fused viscous kernel: u, v, w, T in (32 B) and du[1..4] read and written (64 B); the
+# fluxes never leave the chip
+VISCOUS_BYTES_PER_CELL = 96
 
 
 def parse():
@@ -76,6 +76,10 @@ def parse():
 # op-counting build; tests/test_bench_contract.py keeps the two in step.
 ALGO_FLOPS_PER_CELL = 7431
 ALGO_FLOPS_PER_DIRECTION = 2477
+# The same count with each WENO reconstruction's five (eps + S_j) sums formed once instead of
+# the reference's twenty (3 directions x 10 reconstructions x 15 additions fewer): the lower
+# figure a fused formulation cannot go below, reported beside the headline fraction.
+ALGO_FLOPS_PER_CELL_SHARED_EPS = 6981
 # In the mixed RHS the WENO part (10 x 230 per direction) runs in FP32 and the split and
 # the flux difference (137 + 40) in FP64; its roof is the time of those FLOPs at the
 # two peaks: 2477 / (177 / 78.6 + 2300 / 157.3) = 146.8 TFLOP/s of algorithmic FLOPs.
@@ -298,6 +302,7 @@ def bench_batched(args):
                   autotune_key="tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
                   rk3={"steps": nsteps, "graph": True, "seconds": round(rk3_s, 4),
                        "mcell_updates_per_s": round(N * 3 * nsteps / rk3_s / 1e6, 2),
+                       "mcell_steps_per_s": round(N * nsteps / rk3_s / 1e6, 2),
                        "note": "N x 3 stages x steps / wall time; CFL 0.5 from the per-block minimum each step"})
     print(json.dumps(out))
 
@@ -356,6 +361,7 @@ def bench_rk3(args):
                                "one hipGraph per step", "grid": [nx, ny, nz], "precision": args.precision,
                    "cells": N},
                   roofline=roof, rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
+                  mcell_steps_per_s=round(N * nsteps / secs / 1e6, 2),
                   stage_fusion=args.stage_fusion,
                   totals=[float(v) for v in f.sum(dim=(1, 2, 3)).cpu()])
     out["steps"] = nsteps
@@ -552,7 +558,8 @@ def main():
         torch.cuda.synchronize()
         vis_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
         vis_bytes = VISCOUS_BYTES_PER_CELL * N
-        viscous = {"workload": "inviscid + viscous (CD6, Re 1000, Sutherland) RHS, FP64",
+        viscous = {"workload": "inviscid + viscous (CD6, Re 1000, Sutherland) RHS, FP64; viscous terms in one "
+                               "fused kernel (k_vis_fused)",
                    "value": round(N / (ns_ms * 1e-3) / 1e6, 2), "unit": "Mcell-updates/s",
                    "ms_per_step": round(ns_ms, 4), "viscous_kernels_ms": round(vis_ms, 4),
                    "viscous_roofline": {"bound": "hbm", "achieved": round(vis_bytes / (vis_ms * 1e-3) / 1e9, 1),
@@ -561,11 +568,15 @@ def main():
                                         "bytes_per_cell": VISCOUS_BYTES_PER_CELL}}
         del vctx
 
-    traffic = None
+    # HBM bytes per RHS are not counted in this run: they come from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of the same kernels (scaled by a plain copy's factors)
+    traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", "rhs_traffic_600x256x128.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_rhs")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_rhs")
+        traffic_src = f"profile-derived: profiles/rhs_traffic_600x256x128.json ({tj.get('kernel')})"
 
     out = {
         "metric": METRIC,
@@ -585,10 +596,14 @@ def main():
                    "grid": [nx, ny, nz], "precision": args.precision, "cells": N},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tflops(mixed), 2),
                      "unit": "TFLOP/s", "frac": round(achieved / peak_tflops(mixed), 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_rhs_tile x3 directions (one RHS)",
                      "measured_roof": probe[roof_key],
                      "frac_of_measured": round(achieved / probe[roof_key], 4), "fma_probe": probe,
-                     "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4)},
+                     "algo_flops_per_cell": flops_cell, "kernel_ms": round(kern_ms, 4),
+                     "algo_flops_per_cell_shared_eps": ALGO_FLOPS_PER_CELL_SHARED_EPS,
+                     "frac_shared_eps": round(achieved * ALGO_FLOPS_PER_CELL_SHARED_EPS / flops_cell
+                                              / peak_tflops(mixed), 4)},
         "phases_ms": {"ghost_fill+sound_speed": round(prep_ms, 4), "rhs_kernels": round(kern_ms, 4)},
         "tile_shape/schedule/order": shape, "autotune_ms": tune,
         "autotune_key": "tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch), "

@@ -27,6 +27,11 @@ extern "C" {
  * src/solver/OCFD_NS_Solver.cpp:66-74) are seen.  metrics may be NULL (constant
  * metrics from the config).  `stream` is used by the functions that take none. */
 int ws_ref_bind(ws_ctx *ctx, ws_state *st, const ws_metric_fields *metrics, hipStream_t stream);
+/* The ghosted copy of f that stage 3 of OCFD_time_advance also writes (pf_lap_d,
+ * src/solver/OCFD_time.cpp:48-58; the filter step reads it, src/runtime/OCFD_filtering.cpp:94-176):
+ * a five-variable SoA over the ghosted box, pitch >= nx + 2*LAP (smaller pitches are refused).
+ * NULL or a null ptr: stage 3 writes no copy (the default after ws_ref_bind). */
+int ws_ref_bind_f_lap(const ws_soa *f_lap);
 void ws_ref_set_abort_on_error(int abort_on_error);
 int ws_ref_last_status(void);
 

@@ -53,6 +53,8 @@ def lib():
         L.orc_cd6.argtypes = [_dp]
         L.orc_cd8.restype = ctypes.c_double
         L.orc_cd8.argtypes = [_dp]
+        L.orc_cd_closure.restype = ctypes.c_int
+        L.orc_cd_closure.argtypes = [_dp, ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -160,6 +162,15 @@ def cd(s):
     """Central first-derivative stencil (CD6 for 7 points, CD8 for 9), before 1/h."""
     s = np.ascontiguousarray(s, dtype=np.float64)
     return lib().orc_cd6(_ptr(s)) if len(s) == 7 else lib().orc_cd8(_ptr(s))
+
+
+def cd_closure(s, c, n, lo=1, hi=1):
+    """Closure form of the viscous derivative at cell c of n (s: the 2r+1 values f(c-r..c+r),
+    r = 3 or 4), before 1/h; None where the central form applies."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    out = ctypes.c_double()
+    ok = lib().orc_cd_closure(_ptr(s), (len(s) - 1) // 2, c, n, lo, hi, ctypes.byref(out))
+    return out.value if ok else None
 
 
 def closure(side, line, f, lo=1, hi=1):

@@ -577,7 +577,32 @@ void orc_amu(const orc_params *p, double re, double ref_t, const double *T, doub
             }
 }
 
-/* central derivative of ghosted field f along dir at interior cell (i,j,k), divided by h */
+/* One-sided and low-order forms of the viscous derivatives in the four cells next to a closed
+ * domain face (D0_bound): OCFD_D0bound_scheme_kernel (src/boundary/OCFD_bound_Scheme.cpp:82-263),
+ * which OCFD_CD6_kernel / OCFD_CD8_kernel (src/kernels/OCFD_Schemes.cpp:106-160) consult before
+ * the central form; the flags come from OCFD_bound (:432-452).  s(q) = f(c + q); c is the cell's
+ * index along the direction, n the extent.  The low face's cases are checked first.  Returns 1
+ * and the undivided derivative in *tmp when a form covers the cell, else 0. */
+int orc_cd_closure(const double *s0, int r, long c, long n, int lo, int hi, double *tmp)
+{
+    const double *s = s0 + r;   /* s[q] = f(c + q), q in [-r, r] */
+    if (lo) {
+        if (c == 0) { *tmp = (s[1] - s[0]); return 1; }
+        if (c == 1) { *tmp = (s[1] - s[-1]) * 0.5; return 1; }
+        if (c == 2) { *tmp = (s[-2] - 8.0 * s[-1] + 8.0 * s[1] - s[2]) / 12.0; return 1; }
+        if (c == 3) { *tmp = (s[3] - s[-3] - 9.0 * (s[2] - s[-2]) + 45.0 * (s[1] - s[-1])) / 60.0; return 1; }
+    }
+    if (hi) {
+        if (c == n - 1) { *tmp = (s[0] - s[-1]); return 1; }
+        if (c == n - 2) { *tmp = (s[1] - s[-1]) * 0.5; return 1; }
+        if (c == n - 3) { *tmp = (s[-2] - 8.0 * s[-1] + 8.0 * s[1] - s[2]) / 12.0; return 1; }
+        if (c == n - 4) { *tmp = (s[3] - s[-3] - 9.0 * (s[2] - s[-2]) + 45.0 * (s[1] - s[-1])) / 60.0; return 1; }
+    }
+    return 0;
+}
+
+/* central derivative of ghosted field f along dir at interior cell (i,j,k), divided by h (the
+ * order's central form, OCFD_kernel_CD6/_CD8, or a closure form next to a closed face) */
 static double cd_at(const orc_params *p, int order, int dir, const double *f, long i, long j, long k)
 {
     double s[9];
@@ -585,10 +610,14 @@ static double cd_at(const orc_params *p, int order, int dir, const double *f, lo
     int q;
     for (q = -r; q <= r; q++) {
         const long a = i + (dir == 0 ? q : 0), b = j + (dir == 1 ? q : 0), c = k + (dir == 2 ? q : 0);
-        s[q + r] = f[G(p, a, b, c)];
+        s[q + r] = f[G(p, a, b, c)];   /* the closure forms read no point past a closed face */
     }
     const double h = dir == 0 ? p->hx : dir == 1 ? p->hy : p->hz;
-    const double tmp = order == 8 ? orc_cd8(s) : orc_cd6(s);
+    const long n = dir == 0 ? p->nx : dir == 1 ? p->ny : p->nz;
+    const long c = dir == 0 ? i : dir == 1 ? j : k;
+    double tmp;
+    if (!orc_cd_closure(s, r, c, n, p->d0_bound[2 * dir], p->d0_bound[2 * dir + 1], &tmp))
+        tmp = order == 8 ? orc_cd8(s) : orc_cd6(s);
     return tmp / h;
 }
 

@@ -92,6 +92,10 @@ void orc_rk3_run(const orc_params *p, int nsteps, double dt,
  * divides by h (put_d0_kernel, :80-97). */
 double orc_cd6(const double *s7);
 double orc_cd8(const double *s9);
+/* Closure forms of the viscous derivatives at closed faces: s[0..2r] = f(c-r .. c+r) (r = 3 or
+ * 4), c the index along the direction, n the extent, lo/hi the D0_bound flags of the two faces.
+ * Returns 1 and the undivided derivative in *tmp when a form covers cell c. */
+int orc_cd_closure(const double *s, int r, long c, long n, int lo, int hi, double *tmp);
 
 /* Sutherland viscosity on the interior (get_Amu_kernal, src/kernels/commen_kernel.cpp:123-141):
  * amu = amu_C0 sqrt(T^3)/(Tsb + T), amu_C0 = (1 + 110.4/Ref_T)/Re, Tsb = 110.4/Ref_T

@@ -12,6 +12,7 @@ def test_algorithmic_flop_constant_matches_op_count(orc):
     oc = orc.opcount()
     assert bench.ALGO_FLOPS_PER_CELL == oc["ALGO_FLOPS_PER_CELL"]
     assert bench.ALGO_FLOPS_PER_DIRECTION == oc["per_direction"]
+    assert bench.ALGO_FLOPS_PER_CELL_SHARED_EPS == oc["ALGO_FLOPS_PER_CELL_SHARED_EPS"]
 
 
 def test_multi_gpu_request_exits_with_message():

@@ -27,8 +27,8 @@ def _extents(seed, n, lo, hi):
     return [(rng.randint(lo, hi), rng.randint(lo, hi), rng.randint(lo, hi)) for _ in range(n)]
 
 
-def _batch(extents, seed, precision=W.WS_PREC_FP64, kernel=W.WS_KERNEL_TILED, **kw):
-    prims = [random_state(*e, seed + i, KINDS[i % len(KINDS)]) for i, e in enumerate(extents)]
+def _batch(extents, seed, precision=W.WS_PREC_FP64, kernel=W.WS_KERNEL_TILED, kinds=KINDS, **kw):
+    prims = [random_state(*e, seed + i, kinds[i % len(kinds)]) for i, e in enumerate(extents)]
     states = [upload_state(p) for p in prims]
     base = W.default_config(8, 8, 8, precision=precision, kernel=kernel, **kw)
     return W.Batch(base, extents, states), prims
@@ -128,26 +128,36 @@ def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision, schedule):
         assert rel_l2(st.interior("f").cpu().numpy(), ref.interior("f").cpu().numpy()) <= 1e-12, e
 
 
+@pytest.mark.parametrize("cfl", [-1.0, 0.5])
 @pytest.mark.parametrize("graph", [0, 1])
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
-def test_batch_stage_fusion_levels(ws, precision, graph):
-    """Batched RK3 steps (fixed dt, boundary planes on) with the z launch applying each
-    stage (level 1) and also deriving the primitives of stages 1 and 2 into every block's
-    second primitive set (level 2, the default).  Levels 2 and 1 run the same kernels, so
-    f and every ghosted primitive field agree bit for bit.  Against the separate stage
-    kernels (level 0) the stage-applying z launch is another instantiation of the tile
-    code, which the compiler may contract differently (fused multiply-adds in the WENO and
-    split arithmetic): the states agree to rounding, 1e-14 relative L2 after 3 steps."""
+def test_batch_stage_fusion_levels(ws, precision, graph, cfl):
+    """Batched RK3 steps (boundary planes on; a fixed dt, or dt from the CFL step at CFL
+    0.5) with the z launch applying each stage (level 1) and also deriving the primitives
+    of stages 1 and 2 into every block's second primitive set (level 2, the default).
+    Levels 2 and 1 run the same kernels, so f and every ghosted primitive field agree bit
+    for bit.  Against the separate stage kernels (level 0) the stage-applying z launch is
+    another instantiation of the tile code, which the compiler may contract differently
+    (fused multiply-adds in the WENO and split arithmetic): the states agree to rounding,
+    1e-14 relative L2 after 3 steps in FP64.  The mixed form rounds the window's FP64
+    differences to FP32, so a last-bit difference of the two kernels' split values can flip
+    one of those roundings and move that face's FP32 correction by an FP32 unit of the
+    window's variation: its states agree to 1e-9.  Every state stays finite.  The CFL runs
+    leave out the
+    white-noise ("rough") block: at CFL 0.5 its temperature turns negative within one step
+    in the oracle too (tests/test_oracle_rhs.py), so it is advanced at the small fixed dt
+    only."""
     extents = _extents(19, 5, 6, 30)
+    kinds = KINDS if cfl < 0 else [k for k in KINDS if k != "rough"]
     runs = []
     for level in (0, 1, 2):
-        b, _ = _batch(extents, 700, precision=precision, bc_planes=1, dt=2e-4)
+        b, _ = _batch(extents, 700, precision=precision, kinds=kinds, bc_planes=1, dt=2e-4)
         W.check(W.lib().ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
         W.check(W.lib().ws_batch_set_stage_fusion(b.handle, level))
         for e, st in zip(extents, b.states):
             _to_cons(make_ctx(*e), st)
-        log(f"batch fusion {level} prec {precision} graph {graph}")
-        W.check(W.lib().ws_batch_rk3_run(b.handle, 3, -1.0, graph, W.stream_handle()))
+        log(f"batch fusion {level} prec {precision} graph {graph} cfl {cfl}")
+        W.check(W.lib().ws_batch_rk3_run(b.handle, 3, cfl, graph, W.stream_handle()))
         b.sync_states()
         _sync()
         runs.append([{k: st.t[k].cpu().numpy() for k in ("f", "d", "u", "v", "w", "T", "cc")}
@@ -157,10 +167,10 @@ def test_batch_stage_fusion_levels(ws, precision, graph):
         for k in a:
             assert np.array_equal(a[k], c[k]), (i, k)
     for i, (a, c) in enumerate(zip(runs[0], runs[1])):
-        assert np.isfinite(a["f"]).all()
+        assert np.isfinite(a["f"]).all() and np.isfinite(c["f"]).all(), i
         err = rel_l2(c["f"], a["f"])
         log(f"  block {i}: level 1 vs 0 rel L2 {err:.2e}")
-        assert err <= 1e-14, (i, err)
+        assert err <= (1e-14 if precision == W.WS_PREC_FP64 else 1e-9), (i, err)
 
 
 @pytest.mark.parametrize("fusion", [0, 2])

@@ -1,0 +1,197 @@
+"""The two largest workloads at their full sizes, checked through properties that hold at
+any size and against single-block runs and the oracle where those fit.
+
+* Batched multi-block: 48 periodic blocks with extents drawn from U{32..256} (seed 1),
+  156,710,771 cells, each a freestream of Mach U[0.3, 3] in a seeded direction with
+  Gaussian blobs (the batched benchmark's state).  Every block's RHS finite and
+  conservative; four sampled blocks equal to single-box runs of the same tile code (1e-13
+  relative L2); the two smallest blocks against the oracle (FP64 1e-12, mixed 1e-6); a
+  uniform state gives exactly zero in every block; three graph-replayed RK3 steps equal
+  eager steps bit for bit.
+* Single-GPU share: 1200x256x256 (78.6 M cells), Mach 2 blobs with a Mach 2 planar shock.
+  Tiled against anchor RHS (1e-12 relative L2), conservation, uniform -> 0, bitwise shift
+  equivariance; five graph-replayed RK3 steps (CFL 0.5) finite with the totals of mass,
+  momentum and energy conserved to 1e-12 of sum |f| in FP64 and in mixed precision.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import windstencil_amd as W
+from windstencil_amd import initial as I
+from tests.gpu_util import gpu_du_comput, log, make_ctx, oracle_du, upload_state
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SAMPLED = (0, 17, 33, 47)
+
+
+def _sh():
+    return W.stream_handle()
+
+
+def _to_cons(st):
+    nx, ny, nz = st.n
+    ctx = W.Context(W.default_config(nx, ny, nz))
+    W.check(W.lib().ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                                   W.job((0, 0, 0), (nx, ny, nz)), _sh()), "pri_to_cons")
+
+
+def _gpu_rel_l2(a, b):
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def _conservative(du, tol=1e-12):
+    """sum over the periodic box of every component's RHS vanishes (constant metrics)."""
+    for m in range(5):
+        s, a = float(du[m].sum()), float(du[m].abs().sum())
+        if not abs(s) <= tol * a:
+            return False
+    return True
+
+
+# ---- batched multi-block: 48 blocks, 156.7 M cells ------------------------------------------
+
+@pytest.fixture(scope="module")
+def blocks48():
+    ext = I.multiblock_draw(1)
+    log(f"48 blocks: {sum(a * b * c for a, b, c in ext)} cells; building states")
+    prims = [I.block_freestream(*e, 1000 + i) for i, e in enumerate(ext)]
+    return ext, prims
+
+
+def _batch48(ext, prims, precision):
+    states = []
+    for e, p in zip(ext, prims):
+        st = W.TorchState(*e)
+        st.set_prims(*p)
+        _to_cons(st)
+        states.append(st)
+    return W.Batch(W.default_config(8, 8, 8, precision=precision), ext, states)
+
+
+@pytest.mark.timeout(900)
+def test_batched_48_blocks_rhs(orc, ws, blocks48):
+    ext, prims = blocks48
+    assert len(ext) == 48 and sum(a * b * c for a, b, c in ext) == 156710771
+    smallest = sorted(range(48), key=lambda i: ext[i][0] * ext[i][1] * ext[i][2])[:2]
+    refs = {i: oracle_du(orc, prims[i]) for i in smallest}
+    for prec, tol in ((W.WS_PREC_FP64, 1e-12), (W.WS_PREC_MIXED, 1e-6)):
+        b = _batch48(ext, prims, prec)
+        shape = ctypes.c_int32()
+        W.check(W.lib().ws_batch_get_tile_shape(b.handle, ctypes.byref(shape)))
+        W.check(W.lib().ws_batch_du_comput(b.handle, _sh()), "batch du_comput")
+        torch.cuda.synchronize()
+        log(f"precision {prec}: batched RHS of 48 blocks done (tile shape {shape.value})")
+        for i, st in enumerate(b.states):
+            du = st.interior("du")
+            assert bool(torch.isfinite(du).all()), i
+            assert _conservative(du), i
+        for i in smallest:
+            err = rel_l2(b.states[i].interior("du").cpu().numpy(), refs[i])
+            log(f"  block {i} {ext[i]} against the oracle: rel L2 {err:.3e}")
+            assert err <= tol
+        for i in SAMPLED:
+            ctx = make_ctx(*ext[i], precision=prec)
+            ctx.set_tile_shape(shape.value)
+            ref = torch.from_numpy(gpu_du_comput(ctx, upload_state(prims[i]))).cuda()
+            err = _gpu_rel_l2(b.states[i].interior("du"), ref)
+            log(f"  block {i} {ext[i]} against its single-box run: rel L2 {err:.3e}")
+            assert err <= 1e-13
+        if prec == W.WS_PREC_FP64:
+            # a uniform state (different in every block) gives exactly zero everywhere
+            for i, st in enumerate(b.states):
+                for k, v in zip(("d", "u", "v", "w", "T"), (1.0 + 0.01 * i, 0.3, -0.2, 0.1, 1.2)):
+                    st.interior(k).fill_(v)
+            W.check(W.lib().ws_batch_du_comput(b.handle, _sh()), "batch du_comput")
+            torch.cuda.synchronize()
+            for i, st in enumerate(b.states):
+                assert bool((st.interior("du") == 0.0).all()), i
+        b.close()
+        del b
+        torch.cuda.empty_cache()
+
+
+@pytest.mark.timeout(900)
+def test_batched_48_blocks_graph_steps_equal_eager(ws, blocks48):
+    """Four RK3 steps at CFL 0.5 (the first eager, three replayed from the two captured
+    graphs) against four eager steps: every block's f bit for bit."""
+    ext, prims = blocks48
+    runs = []
+    for graph in (1, 0):
+        b = _batch48(ext, prims, W.WS_PREC_FP64)
+        W.check(W.lib().ws_batch_rk3_run(b.handle, 4, 0.5, graph, _sh()), "batch rk3 run")
+        b.sync_states()
+        torch.cuda.synchronize()
+        log(f"48 blocks: 4 RK3 steps, graph {graph}")
+        runs.append([st.interior("f").clone() for st in b.states])
+        b.close()
+        del b
+        torch.cuda.empty_cache()
+    for i, (a, c) in enumerate(zip(*runs)):
+        assert bool(torch.isfinite(a).all()), i
+        assert torch.equal(a, c), i
+
+
+# ---- single-GPU share: 1200x256x256 ------------------------------------------------------------
+
+N5 = (1200, 256, 256)
+
+
+@pytest.fixture(scope="module")
+def shock_box():
+    log("1200x256x256: Mach 2 blobs + planar shock; building state")
+    return I.blobs_with_shock(*N5, 7)
+
+
+def _du_comput(ctx, st):
+    W.check(W.lib().ws_du_comput(ctx.handle, ctypes.byref(st.ws), None, _sh()), "du_comput")
+    torch.cuda.synchronize()
+    return st.interior("du")
+
+
+@pytest.mark.timeout(900)
+def test_share_1200x256x256_rhs_properties(ws, shock_box):
+    nx, ny, nz = N5
+    st = upload_state(shock_box)
+    ctx = make_ctx(nx, ny, nz)
+    du = _du_comput(ctx, st).clone()
+    assert bool(torch.isfinite(du).all())
+    assert _conservative(du)
+    da = _du_comput(make_ctx(nx, ny, nz, kernel=W.WS_KERNEL_ANCHOR), st)
+    err = _gpu_rel_l2(du, da)
+    log(f"  tiled vs anchor rel L2 {err:.3e}")
+    assert err <= 1e-12
+    # a periodic shift of the input by (3, 5, 7) cells shifts the RHS bit for bit
+    for k in ("d", "u", "v", "w", "T"):
+        v = st.interior(k)
+        v.copy_(torch.roll(v.clone(), shifts=(3, 5, 7), dims=(0, 1, 2)))
+    dr = _du_comput(ctx, st)
+    assert torch.equal(dr, torch.roll(du, shifts=(3, 5, 7), dims=(1, 2, 3)))
+    for k, v in zip(("d", "u", "v", "w", "T"), (1.1, 0.4, -0.3, 0.2, 1.3)):
+        st.interior(k).fill_(v)
+    assert bool((_du_comput(ctx, st) == 0.0).all())
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+def test_share_1200x256x256_graph_steps_conserve(ws, shock_box, precision):
+    nx, ny, nz = N5
+    st = upload_state(shock_box)
+    _to_cons(st)
+    torch.cuda.synchronize()
+    f0 = st.interior("f")
+    tot0 = f0.sum(dim=(1, 2, 3))
+    scale = f0.abs().sum(dim=(1, 2, 3))
+    ctx = make_ctx(nx, ny, nz, precision=precision)
+    W.check(W.lib().ws_rk3_run(ctx.handle, ctypes.byref(st.ws), 5, 0.5, 1, _sh()), "rk3 run")
+    st.sync_from_ws()
+    torch.cuda.synchronize()
+    f = st.interior("f")
+    assert bool(torch.isfinite(f).all())
+    drift = ((f.sum(dim=(1, 2, 3)) - tot0).abs() / scale).cpu().numpy()
+    log(f"  precision {precision}: totals drift / sum|f| per component {drift}")
+    assert np.all(drift <= 1e-12)

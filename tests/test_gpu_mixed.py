@@ -160,3 +160,23 @@ def test_mixed_sod_600x256x128_rhs_and_200_steps(ws):
         log(f"    component {m}: rel L2 {float(torch.linalg.vector_norm(fb[m] - fa[m]) / torch.linalg.vector_norm(fa[m])):.3e}"
             f"  Linf {float(dm.max()):.3e}  max|f| {float(fa[m].abs().max()):.3e}")
     assert l2 <= TOL_MIXED and linf <= TOL_MIXED
+
+
+@pytest.mark.parametrize("ratio", [1e1, 1e2, 1e3, 1e4, 1e6])
+def test_mixed_rhs_at_strong_discontinuities(orc, ws, ratio):
+    """Spherical blasts with pressure ratios 1e1..1e6 (split-flux jumps of about 1e1..1e9
+    per cell): the FP32 weights of the faces next to the jump leave FP32's range from a
+    ratio of about 1e3 on (tests/test_mixed_range.py), those faces fall back to FP64, and
+    the mixed RHS stays finite and within the mixed bar of the FP64 oracle."""
+    from windstencil_amd import initial as I
+    nx, ny, nz = 48, 20, 16
+    prims = I.blast(nx, ny, nz, 3, ratio)
+    ref = oracle_du(orc, prims)
+    for tile in (W.WS_TILE_30x16_K2, W.WS_TILE_126x2_K2, W.WS_TILE_15x16_K1):
+        ctx = _mixed_ctx(nx, ny, nz)
+        ctx.set_tile_shape(tile)
+        du = gpu_du_comput(ctx, upload_state(prims))
+        err = rel_l2(du, ref)
+        log(f"blast ratio {ratio:g} tile {tile}: finite {np.isfinite(du).all()} rel L2 {err:.3e}")
+        assert np.isfinite(du).all()
+        assert err <= TOL_MIXED

@@ -336,3 +336,25 @@ def test_curvilinear_metrics_from_jacobi_file(orc, ws, tmp_path):
     err = rel_l2(gpu_du_comput(mctx, upload_state(prims), metrics=mf), ref)
     log(f"  curvilinear mixed: rel L2 {err:.3e}")
     assert err <= 1e-6
+
+
+def test_anchor_contexts_on_two_streams_run_concurrently(orc, ws):
+    """Each context owns its anchor scratch: two contexts of different sizes running the
+    anchor RHS on two streams at once (no synchronisation between their launches) both
+    stay bit-identical to the oracle."""
+    import torch
+    L = W.lib()
+    runs = []
+    for (nx, ny, nz), seed, kind in (((40, 24, 20), 3, "shock"), ((33, 28, 17), 4, "rough")):
+        prims = random_state(nx, ny, nz, seed, kind)
+        runs.append((make_ctx(nx, ny, nz, kernel=W.WS_KERNEL_ANCHOR), upload_state(prims), oracle_du(orc, prims),
+                     torch.cuda.Stream()))
+    torch.cuda.synchronize()
+    log("anchor RHS of two contexts interleaved on two streams")
+    for _ in range(3):
+        for ctx, st, _, strm in runs:
+            W.check(L.ws_du_comput(ctx.handle, ctypes.byref(st.ws), None, W.stream_handle(strm)), "du_comput")
+    torch.cuda.synchronize()
+    for _, st, ref, _ in runs:
+        du = st.interior("du").cpu().numpy()
+        assert np.array_equal(du, ref), f"rel L2 {rel_l2(du, ref)}"

@@ -297,3 +297,98 @@ def test_restart_save_and_load_device_state(ws, tmp_path):
         full = st2.t[k].clone()
         full[LAP:LAP + nz, LAP:LAP + ny, LAP:LAP + nx] = 0
         assert float(full.abs().max()) == 0.0
+
+
+def test_mem_value_init_on_pitched_boxes(ws):
+    """cuda_mem_value_init_warp / ws_mem_value_init (commen_kernel.cpp:12-29) as the
+    reference calls it (OCFD_init.cpp:39-48): a five-variable interior state
+    (nx, ny, nz*5) and a ghosted field, both with pitches wider than the rows.  Exactly
+    the box's elements take the value; the row padding keeps what it held."""
+    import torch
+    L = W.lib()
+    nx, ny, nz, pad = 37, 11, 9, 27
+    soa = torch.full((5, nz, ny, nx + pad), 7.0, dtype=torch.float64, device="cuda")
+    L.ws_ref_set_abort_on_error(0)
+    L.cuda_mem_value_init_warp(0.0, ctypes.c_void_p(soa.data_ptr()), nx + pad, nx, ny, nz * 5)
+    assert L.ws_ref_last_status() == 0
+    g = torch.full((nz + 8, ny + 8, nx + 8 + 5), 7.0, dtype=torch.float64, device="cuda")
+    W.check(L.ws_mem_value_init(-2.5, ctypes.c_void_p(g.data_ptr()), nx + 13, nx + 8, ny + 8, nz + 8,
+                                W.stream_handle()))
+    # an empty box writes nothing; a pitch below the row is refused
+    W.check(L.ws_mem_value_init(1.0, ctypes.c_void_p(g.data_ptr()), nx + 13, 0, ny, nz, W.stream_handle()))
+    assert L.ws_mem_value_init(1.0, ctypes.c_void_p(g.data_ptr()), 4, 8, 2, 2, W.stream_handle()) == -1
+    L.ws_ref_set_abort_on_error(1)
+    _sync()
+    assert bool((soa[..., :nx] == 0.0).all()) and bool((soa[..., nx:] == 7.0).all())
+    assert bool((g[..., :nx + 8] == -2.5).all()) and bool((g[..., nx + 8:] == 7.0).all())
+
+
+@pytest.mark.parametrize("kernel", [W.WS_KERNEL_TILED, W.WS_KERNEL_ANCHOR])
+def test_du_invis_jacobian3d_x_by_name_on_a_sub_box(orc, ws, kernel):
+    """The reference-named du_invis_Jacobian3d_init / du_invis_Jacobian3d_x
+    (OCFD_NS_Jacobian3d.h:11-15) on a sub-box job: du over that box equals the oracle's
+    RHS there (anchor kernels bit for bit, tiled within 1e-12 relative L2) and every cell
+    outside the box keeps what du held."""
+    import torch
+    from tests.gpu_util import oracle_du
+    nx, ny, nz = 41, 26, 19
+    prims = random_state(nx, ny, nz, 12, "shock")
+    ref = oracle_du(orc, prims)
+    ctx = make_ctx(nx, ny, nz, kernel=kernel)
+    st = upload_state(prims)
+    st.t["du"].fill_(-9.0)
+    L = W.lib()
+    s = W.stream_handle()
+    W.check(L.ws_ghost_fill_periodic(ctx.handle, (W.WsField * 5)(st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T), 5, s))
+    L.ws_ref_set_abort_on_error(0)
+    W.check(L.ws_ref_bind(ctx.handle, ctypes.byref(st.ws), None, s))
+    lo, hi = (6, 3, 2), (35, 20, 17)                     # interior cells [lo, hi)
+    job = W.job(tuple(LAP + v for v in lo), tuple(LAP + v for v in hi))
+    sp = ctypes.c_void_p(s.value)
+    L.du_invis_Jacobian3d_init(job, ctypes.byref(sp))
+    L.du_invis_Jacobian3d_x(job, None, None, ctypes.byref(sp))
+    assert L.ws_ref_last_status() == 0
+    L.ws_ref_set_abort_on_error(1)
+    _sync()
+    du = st.interior("du").cpu().numpy()
+    box = (slice(None), slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+    if kernel == W.WS_KERNEL_ANCHOR:
+        assert np.array_equal(du[box], ref[box])
+    else:
+        err = rel_l2(du[box], ref[box])
+        log(f"  sub-box rel L2 {err:.3e}")
+        assert err <= 1e-12
+    mask = np.ones(du.shape, bool)
+    mask[box] = False
+    assert np.all(du[mask] == -9.0)
+
+
+def test_time_advance_writes_the_bound_f_lap(ws):
+    """OCFD_time_advance(3) with an f_lap bound (ws_ref_bind_f_lap) writes the new f into
+    its interior as the reference's stage 3 does (OCFD_time.cpp:48-58); a pitch below the
+    ghosted row is refused."""
+    import torch
+    nx, ny, nz = 22, 13, 10
+    L = W.lib()
+    s = W.stream_handle()
+    ctx = make_ctx(nx, ny, nz, dt=1e-3)
+    st = upload_state(random_state(nx, ny, nz, 8, "blobs"))
+    for k, v in (("f", 1.0), ("fn", 2.0), ("du", 3.0)):
+        st.t[k].copy_(torch.rand_like(st.t[k]) + v)
+    lap = torch.full((5, nz + 8, ny + 8, nx + 8 + 3), 5.0, dtype=torch.float64, device="cuda")
+    soa = W.WsSoa(lap.data_ptr(), nx + 11, ny + 8, nz + 8, 0)
+    W.check(L.ws_ref_bind(ctx.handle, ctypes.byref(st.ws), None, s))
+    W.check(L.ws_ref_bind_f_lap(ctypes.byref(soa)))
+    L.ws_ref_set_abort_on_error(0)
+    L.OCFD_time_advance(3)
+    assert L.ws_ref_last_status() == 0
+    L.ws_ref_set_abort_on_error(1)
+    _sync()
+    inner = lap[:, LAP:LAP + nz, LAP:LAP + ny, LAP:LAP + nx]
+    assert torch.equal(inner, st.interior("f"))
+    mask = torch.ones_like(lap, dtype=torch.bool)
+    mask[:, LAP:LAP + nz, LAP:LAP + ny, LAP:LAP + nx] = False
+    assert bool((lap[mask] == 5.0).all())
+    bad = W.WsSoa(lap.data_ptr(), nx + 2, ny + 8, nz + 8, 0)
+    assert L.ws_ref_bind_f_lap(ctypes.byref(bad)) == -1
+    W.check(L.ws_ref_bind_f_lap(None))

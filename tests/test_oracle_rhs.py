@@ -176,3 +176,22 @@ def test_rk3_run_conserves_totals(orc):
     assert np.all(np.isfinite(f))
     # the boundary-plane prim->cons round trip perturbs totals at the ulp level only
     np.testing.assert_allclose(tot1, tot0, rtol=1e-12, atol=1e-10)
+
+
+def test_white_noise_block_loses_positivity_at_cfl_half(orc):
+    """The batched fusion test's white-noise block (kind "rough", 19x9x25, seed 704:
+    standard-normal velocities at every cell) is not advanced stably by SSP-RK3 with
+    WENO7/Steger-Warming at CFL 0.5: in the oracle its temperature turns NaN within the
+    first step (the sound speed sqrt(T) of a negative T), so a NaN of a GPU run of that
+    block at CFL 0.5 is the workload's, not a kernel's.  At the small fixed dt of the
+    fixed-step tests (2e-4, a twentieth of the CFL step) it stays finite."""
+    nx, ny, nz = 19, 9, 25
+    prims = random_state(nx, ny, nz, 704, "rough")
+    for dt, finite in ((-0.5, False), (2e-4, True)):
+        p = orc.params(nx, ny, nz)
+        g = [np.ascontiguousarray(ghosted(a)) for a in prims]
+        for a in g:
+            orc.ghost_fill_periodic(p, a)
+        f = np.ascontiguousarray(orc.pri_to_cons(p, *g))
+        orc.rk3_run(p, 1 if dt < 0 else 3, dt, f, *g)
+        assert bool(np.isfinite(f).all()) == finite, dt

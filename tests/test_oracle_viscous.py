@@ -94,3 +94,53 @@ def test_c_oracle_matches_tensor_form(order):
     dn = N.viscous_rhs((u, v, w, T), met, aj, (p.hx, p.hy, p.hz), 1.4, 1.0, RE, PR, TREF, order)
     assert np.linalg.norm(dc - dn) <= 1e-13 * np.linalg.norm(dn)
     assert np.all(dc[0] == 0.0)
+
+
+@pytest.mark.parametrize("r", [3, 4])
+def test_closure_forms_exact_on_polynomials(r):
+    """The forms next to a closed face (OCFD_bound_Scheme.cpp:82-263): at distance t from the
+    face a one-sided first-order difference (t = 0), a second-order (t = 1), a fourth-order
+    (t = 2) and a sixth-order (t = 3) central difference; each exact up to its degree and not
+    beyond, on both faces; interior cells and open faces fall through to the central form."""
+    n, h = 12, 0.07
+    degree = {0: 1, 1: 2, 2: 4, 3: 6}
+    for t in range(4):
+        for c, lo, hi in ((t, 1, 0), (n - 1 - t, 0, 1)):
+            x0 = 0.2 + c * h
+            x = x0 + h * np.arange(-r, r + 1)
+            for k in range(degree[t] + 2):
+                got = O.cd_closure(x ** k, c, n, lo, hi)
+                assert got is not None
+                exact = k * x0 ** (k - 1) if k else 0.0
+                err = abs(got / h - exact)
+                if k <= degree[t]:
+                    assert err <= 1e-10 * max(1.0, abs(exact)), (r, t, c, k)
+                else:
+                    assert err > 1e-8, (r, t, c, k)
+            assert O.cd_closure(x, c, n, 1 - lo, 1 - hi) is None
+    for c in range(4, n - 4):
+        assert O.cd_closure(np.arange(2 * r + 1.0), c, n) is None
+
+
+@pytest.mark.parametrize("order", [6, 8])
+def test_closed_box_linear_velocity_closed_form(order):
+    """A box closed on all six faces (D0_bound everywhere, no ghosts filled: the ghost
+    layers hold zeros), a linear velocity field u_i = G_ij x_j and T = 1 (mu = 1/Re): every
+    derivative form is exact on linear data, the stress is uniform, so du1..du3 = 0 and du4 =
+    sigma : grad u (the viscous dissipation) in every cell, boundary cells included."""
+    nx, ny, nz = 13, 11, 10
+    p = O.params(nx, ny, nz, periodic=(0, 0, 0), d0_bound=(1,) * 6)
+    G = np.array([[0.3, -0.2, 0.1], [0.15, 0.25, -0.05], [-0.1, 0.2, 0.35]])
+    k, j, i = np.meshgrid(np.arange(nz) * p.hz, np.arange(ny) * p.hy, np.arange(nx) * p.hx, indexing="ij")
+    X = (i, j, k)
+    vel = [sum(G[a, b] * X[b] for b in range(3)) for a in range(3)]
+    g = [ghosted(a) for a in vel + [np.ones((nz, ny, nx))]]
+    du = O.viscous_rhs(p, *g, RE, PR, TREF, order)
+    mu = 1.0 / RE
+    div = np.trace(G)
+    sig = mu * (G + G.T) - 2.0 / 3.0 * mu * div * np.eye(3)
+    diss = float(np.sum(sig * G))
+    assert np.all(du[0] == 0.0)
+    for m in (1, 2, 3):
+        assert np.abs(du[m]).max() <= 1e-10 * abs(diss)
+    assert np.abs(du[4] - diss).max() <= 1e-10 * abs(diss)

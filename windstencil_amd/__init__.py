@@ -166,6 +166,7 @@ _SIGS = {
     # reference-named surface (include/windstencil_amd_ref.h)
     "ws_ref_bind": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState), ctypes.POINTER(WsMetricFields),
                                    ctypes.c_void_p]),
+    "ws_ref_bind_f_lap": (ctypes.c_int, [ctypes.POINTER(WsSoa)]),
     "ws_ref_set_abort_on_error": (None, [ctypes.c_int]),
     "ws_ref_last_status": (ctypes.c_int, []),
     "du_invis_Jacobian3d_init": (None, [WsJob, ctypes.POINTER(ctypes.c_void_p)]),

@@ -150,8 +150,12 @@ int validate_config(const ws_config *cfg)
         return set_error(WS_ERR_INVALID, "viscous must be 0 (off), 6 or 8 (central-difference order)");
     if (cfg->viscous && !(cfg->re > 0.0 && cfg->pr > 0.0 && cfg->ref_t > 0.0))
         return set_error(WS_ERR_INVALID, "viscous terms need positive Re, Pr and Ref_T");
-    if (cfg->viscous && !(cfg->periodic[0] && cfg->periodic[1] && cfg->periodic[2]))
-        return set_error(WS_ERR_UNSUPPORTED, "the viscous terms are built for periodic directions only");
+    // the viscous derivatives need a periodic direction or closures on both of its faces (the
+    // flux fields get no ghost layers from the caller)
+    for (int i = 0; i < 3 && cfg->viscous; ++i)
+        if (!cfg->periodic[i] && !(cfg->d0_bound[2 * i] && cfg->d0_bound[2 * i + 1]))
+            return set_error(WS_ERR_UNSUPPORTED,
+                             "viscous terms: every direction must be periodic or closed on both faces (d0_bound)");
     return WS_OK;
 }
 
@@ -401,8 +405,9 @@ int ws_ctx_destroy(ws_ctx *c)
     if (!c) return WS_OK;
     if (c->capture_stream) hipStreamDestroy(c->capture_stream);
     if (c->cfl_partial) hipFree(c->cfl_partial);
-    if (c->vis_scratch || c->prim_scratch || c->cfl_stage) hipDeviceSynchronize();
+    if (c->vis_scratch || c->prim_scratch || c->cfl_stage || c->anchor_scratch) hipDeviceSynchronize();
     if (c->vis_scratch) hipFree(c->vis_scratch);
+    if (c->anchor_scratch) hipFree(c->anchor_scratch);
     if (c->prim_scratch) hipFree(c->prim_scratch);
     if (c->cfl_stage) hipFree(c->cfl_stage);
     delete c;
@@ -640,6 +645,10 @@ int ws_time_advance(ws_ctx *c, int32_t krk, ws_soa f, ws_soa fn, ws_soa du, ws_s
     if (krk < 1 || krk > 3) return set_error(WS_ERR_INVALID, "krk must be 1, 2 or 3");
     int r;
     if ((r = check_soa(c, f, "f")) || (r = check_soa(c, fn, "fn")) || (r = check_soa(c, du, "du"))) return r;
+    // f_lap is indexed over the ghosted box (x + LAP + pitch * (...)): a pitch below the
+    // ghosted row would write into the next row
+    if (f_lap.ptr && f_lap.pitch < (uint32_t)(c->cfg.nx + 2 * LAP))
+        return set_error(WS_ERR_INVALID, "f_lap: pitch smaller than nx+2*LAP");
     return launch_time_advance(*c, krk, f, fn, du, f_lap, dt, nullptr, s);
 }
 

@@ -143,12 +143,39 @@ __global__ void k_zero_box(ws_soa du, int ny, int nz, Box box)
     for (int m = 0; m < 5; ++m) du.ptr[sidx(du.pitch, ny, nz, x, y, z, m)] = 0.0;
 }
 
-double *g_scratch = nullptr;
-size_t g_scratch_bytes = 0;
-
 }  // namespace
 
-int launch_rhs_anchor(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+// The context's split-flux scratch of the anchor kernels: 10 doubles per point of the
+// largest directional sweep of the whole interior, so that every job box of the context
+// fits and the buffer never grows (a graph captured around an anchor RHS keeps its
+// pointer valid until ws_ctx_destroy).  Allocated by the first anchor RHS of the context,
+// which must not run under stream capture.
+static int anchor_scratch(Ctx &c, hipStream_t s, double **out)
+{
+    if (!c.anchor_scratch) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+            return set_error(WS_ERR_UNSUPPORTED,
+                             "anchor RHS: its scratch is allocated by the context's first anchor call, which "
+                             "must run outside a stream capture");
+        const size_t n[3] = {(size_t)c.kp.nx, (size_t)c.kp.ny, (size_t)c.kp.nz};
+        size_t need = 0;
+        for (int dir = 0; dir < 3; ++dir) {
+            size_t pts = n[dir] + 2 * LAP;
+            for (int ax = 0; ax < 3; ++ax)
+                if (ax != dir) pts *= n[ax];
+            need = need > pts ? need : pts;
+        }
+        need *= 10 * sizeof(double);
+        hipError_t e = hipMalloc(&c.anchor_scratch, need);
+        if (e != hipSuccess) { c.anchor_scratch = nullptr; return hip_error(e, "anchor: scratch allocation"); }
+        c.anchor_scratch_bytes = need;
+    }
+    *out = c.anchor_scratch;
+    return WS_OK;
+}
+
+int launch_rhs_anchor(Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                       const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s)
 {
     Box box;
@@ -156,27 +183,9 @@ int launch_rhs_anchor(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field
     box.n[0] = (int)(job.end.x - job.start.x);
     box.n[1] = (int)(job.end.y - job.start.y);
     box.n[2] = (int)(job.end.z - job.start.z);
-    size_t need = 0;
-    for (int dir = 0; dir < 3; ++dir) {
-        size_t pts = (size_t)(box.n[dir] + 2 * LAP);
-        for (int ax = 0; ax < 3; ++ax)
-            if (ax != dir) pts *= (size_t)box.n[ax];
-        need = need > pts ? need : pts;
-    }
-    need *= 10 * sizeof(double);
-    if (need > g_scratch_bytes) {
-        // the anchor is a test path: keep one grow-only buffer for the process
-        if (g_scratch) {
-            hipError_t e = hipStreamSynchronize(s);
-            if (e != hipSuccess) return hip_error(e, "anchor: stream sync");
-            hipFree(g_scratch);
-            g_scratch = nullptr;
-            g_scratch_bytes = 0;
-        }
-        hipError_t e = hipMalloc(&g_scratch, need);
-        if (e != hipSuccess) return hip_error(e, "anchor: scratch allocation");
-        g_scratch_bytes = need;
-    }
+    double *scratch = nullptr;
+    int r = anchor_scratch(c, s, &scratch);
+    if (r) return r;
     ws_metric_fields mf{};
     const int fields = m != nullptr;
     if (m) mf = *m;
@@ -190,9 +199,9 @@ int launch_rhs_anchor(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field
         long pts = (long)(box.n[DIR] + 2 * LAP) * box.n[DIR == 0 ? 1 : 0] * box.n[DIR == 2 ? 1 : 2];        \
         long cells = (long)box.n[0] * box.n[1] * box.n[2];                                                  \
         hipLaunchKernelGGL(k_anchor_split<DIR>, dim3((unsigned)((pts + T - 1) / T)), dim3(T), 0, s, c.kp, d, \
-                           u, v, w, cc, mf, fields, box, g_scratch);                                        \
+                           u, v, w, cc, mf, fields, box, scratch);                                        \
         hipLaunchKernelGGL(k_anchor_weno<DIR>, dim3((unsigned)((cells + T - 1) / T)), dim3(T), 0, s, c.kp,  \
-                           mf, fields, box, (const double *)g_scratch, du);                                 \
+                           mf, fields, box, (const double *)scratch, du);                                 \
     }
     WS_ANCHOR_DIR(0)
     WS_ANCHOR_DIR(1)

@@ -14,6 +14,7 @@ struct Bound {
     ws_state *st = nullptr;
     const ws_metric_fields *metrics = nullptr;
     hipStream_t stream = nullptr;
+    ws_soa f_lap{};          // the ghosted copy stage 3 writes (pf_lap_d); ptr null: none
 };
 
 Bound g_bound;
@@ -58,7 +59,23 @@ int ws_ref_bind(ws_ctx *ctx, ws_state *st, const ws_metric_fields *metrics, hipS
     g_bound.st = st;
     g_bound.metrics = metrics;
     g_bound.stream = stream;
+    g_bound.f_lap = ws_soa{};
     g_last_status = WS_OK;
+    return WS_OK;
+}
+
+int ws_ref_bind_f_lap(const ws_soa *f_lap)
+{
+    if (!g_bound.ctx) return ws::set_error(WS_ERR_INVALID, "ws_ref_bind_f_lap: no context bound");
+    if (f_lap && f_lap->ptr) {
+        ws_config c;
+        ws_ctx_get_config(g_bound.ctx, &c);
+        if (f_lap->pitch < (uint32_t)(c.nx + 2 * WS_LAP))
+            return ws::set_error(WS_ERR_INVALID, "ws_ref_bind_f_lap: pitch smaller than nx+2*LAP");
+        g_bound.f_lap = *f_lap;
+    } else {
+        g_bound.f_lap = ws_soa{};
+    }
     return WS_OK;
 }
 
@@ -135,8 +152,7 @@ void OCFD_time_advance(int KRK)
     ws_config c;
     ws_ctx_get_config(g_bound.ctx, &c);
     const ws_state *s = g_bound.st;
-    const ws_soa nolap{};
-    fail(__func__, ws_time_advance(g_bound.ctx, KRK, s->f, s->fn, s->du, nolap, c.dt, g_bound.stream));
+    fail(__func__, ws_time_advance(g_bound.ctx, KRK, s->f, s->fn, s->du, g_bound.f_lap, c.dt, g_bound.stream));
 }
 
 void OCFD_bc(void)

@@ -240,6 +240,8 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     const bool lo = BND && k.bnd[2 * DIR], hi = BND && k.bnd[2 * DIR + 1];
     const bool nr_lo = lo && k.nonref[2 * DIR], nr_hi = hi && k.nonref[2 * DIR + 1];
     double nfix[BND ? 5 : 1];       // P(n) - P(n-1) of the line's last cell (non-reflecting M)
+    // MIX: bit m*K + j set when face j of variable m had FP32 weights out of FP32's range
+    unsigned bad_mix = 0;
     if (active) {
         const int base = TL::pidx(line, f0);
         const bool near = BND && ((lo && gl0 + f0 <= 3) || (hi && gl0 + f0 + K - 1 >= n_dom - 3));
@@ -269,10 +271,12 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                 // FP32 reconstruction relative to FP64 base points; the merge is FP64
                 double bP[K], bM[K];
                 float cP[K], cM[K];
-                weno_k_mixed<false, K>([&](int j) { return qp[j]; }, bP, cP);
-                weno_k_mixed<true, K>([&](int j) { return qm[j]; }, bM, cM);
+                unsigned badP, badM;
+                weno_k_mixed<false, K>([&](int j) { return qp[j]; }, bP, cP, badP);
+                weno_k_mixed<true, K>([&](int j) { return qm[j]; }, bM, cM, badM);
 #pragma unroll
                 for (int j = 0; j < K; ++j) Fv[m][j] = (bP[j] + bM[j]) + (double)(cP[j] + cM[j]);
+                bad_mix |= (badP | badM) << (m * K);
                 if (near)
 #pragma unroll
                     for (int j = 0; j < K; ++j)
@@ -290,6 +294,38 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
 #pragma unroll
                     for (int j = 0; j < K; ++j)
                         closed(j, [&] { return nP[j] / dP[j]; }, [&] { return nM[j] / dM[j]; });
+            }
+        }
+        // Faces whose FP32 weights left FP32's range (strong discontinuities of the split
+        // flux): both halves again in FP64 from the split values still in LDS, merged as the
+        // FP64 kernel merges them (a half a closure covers keeps its closure value).  Done
+        // after the variable loop, when the FP32 registers are dead, and skipped by every
+        // wave without such a face.
+        if (MIX && bad_mix) {
+#pragma unroll 1
+            for (int m = 0; m < 5; ++m) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    if (!((bad_mix >> (m * K + j)) & 1u)) continue;
+                    const double *qp = sp + m * TL::QS + base, *qm = sp + (5 + m) * TL::QS + base;
+                    double nP, dP, nM, dM, val;
+                    const int fg = gl0 + f0 + j;
+                    if (BND && ((lo && fg <= 3) || (hi && fg >= n_dom - 3))) {
+                        auto cellP = [&](int cg) { return sp[m * TL::QS + TL::pidx(line, cg - gl0 + LAP)]; };
+                        auto cellM = [&](int cg) { return sp[(5 + m) * TL::QS + TL::pidx(line, cg - gl0 + LAP)]; };
+                        double pv, mv;
+                        if (!closure_p(cellP, n_dom, fg, lo, hi, pv)) { face64<false>(qp + j, nP, dP); pv = nP / dP; }
+                        if (!closure_m(cellM, n_dom, fg, lo, hi, mv)) { face64<true>(qm + j, nM, dM); mv = nM / dM; }
+                        val = pv + mv;
+                    } else {
+                        face64<false>(qp + j, nP, dP);
+                        face64<true>(qm + j, nM, dM);
+                        val = fma(nP, dM, nM * dP) * recip_nr(dP * dM);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 5; ++q)
+                        if (q == m) Fv[q][j] = val;
+                }
             }
         }
     }

@@ -290,15 +290,37 @@ __device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[
     }
 }
 
+// One face of weno_k from the 8-point window q[0..7]: num / den is the face value.
+template <bool REV>
+__device__ __forceinline__ void face64(const double *q, double &num, double &den)
+{
+    double n[1], d[1];
+    weno_k<REV, 1>([&](int i) { return q[i]; }, n, d);
+    num = n[0];
+    den = d[0];
+}
+
 // Mixed-precision form of weno_k: the window stays FP64 in LDS; its first differences
 // are formed in FP64 and rounded to FP32, and everything after that (second and third
 // differences, smoothness, weights, candidates) runs in FP32 on values taken relative to
 // the face's base point w3.  The face value is base[j] + corr[j]: base in FP64, the FP32
 // correction carries an error relative to the local variation of the window, not to the
 // size of the flux.
+//
+// Range check: the weights are products of four (eps + S_k) factors, homogeneous of degree
+// eight in the window's differences, and the numerator of degree nine.  D is at least
+// d_4 eps^4 (about 5e-36 with eps = 3.6e-9), so it never falls below FP32's normal range,
+// but it overflows once the window's first differences reach a few hundred (a strong
+// discontinuity of the split flux); N may overflow a little earlier.  Bit j of `bad` is set
+// when face j's D or 1/D is not a normal number or its N is not finite: the caller then
+// recomputes that face in FP64 (weno_k), so FP32 weights are used only where they are
+// representable.  Below that limit the FP32 correction's error stays relative to the
+// window's variation (no catastrophic cancellation: the indicators are sums of squares
+// and the weights a convex combination), so no other check is needed.
 template <bool REV, int K, typename Load>
-__device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (&corr)[K])
+__device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (&corr)[K], unsigned &bad)
 {
+    bad = 0;
     constexpr int NP = K + 7;
     constexpr float fD0 = (float)kD0, fD1 = (float)kD1, fD2 = (float)kD2, fD3 = (float)kD3, fD4 = (float)kD4;
     constexpr float fB = 39.0f, fC = 39.05f, fEps = 3.6e-9f;
@@ -361,6 +383,10 @@ __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (
         D = fmaf(D, e4, fD4 * p0123);
         corr[j] = N * __builtin_amdgcn_rcpf(D);
         base[j] = REV ? s[j + 4] : s[j + 3];
+        // D in [2^-126, 2^125], so that D and its reciprocal are normal (false for NaN), and
+        // |N| finite
+        const bool ok = D >= 1.17549435e-38f && D <= 4.25352958e+37f && fabsf(N) <= 3.40282347e+38f;
+        bad |= ok ? 0u : (1u << j);
     }
 }
 

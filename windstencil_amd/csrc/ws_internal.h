@@ -72,9 +72,11 @@ struct Ctx {
     // RK3 steps apply each stage's update in the last RHS launch where they can (1), and
     // there also derive the primitives of stages 1 and 2 (2)
     int fuse_stage = 2;
-    // viscous terms: twelve ghosted flux fields, allocated at the first use
+    // viscous terms of the two-kernel form (anchor family, metric fields): twelve ghosted
+    // flux fields, allocated at the first use; planes per work-group of the fused form
     double *vis_scratch = nullptr;
     size_t vis_scratch_elems = 0;
+    int vis_zchunk = 32;
     // second set of ghosted primitive fields (d, u, v, w, T, cc) that RK3 stages 1 and 2
     // write their primitives to, allocated at the first eager RK3 step that can use it
     double *prim_scratch = nullptr;
@@ -83,6 +85,10 @@ struct Ctx {
     // (one per stage-kernel work-group), allocated by the first eager step
     double *cfl_stage = nullptr;
     long cfl_stage_n = 0;
+    // split fluxes of the anchor kernels (10 doubles per point of the largest directional
+    // sweep of the interior), allocated by the context's first anchor RHS
+    double *anchor_scratch = nullptr;
+    size_t anchor_scratch_bytes = 0;
 };
 
 KParams make_kparams(const ws_config &cfg);
@@ -132,7 +138,7 @@ int set_error(int code, const std::string &msg);
 int hip_error(hipError_t e, const char *what);
 
 // ---- launchers (defined next to their kernels) ----
-int launch_rhs_anchor(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
+int launch_rhs_anchor(Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                       const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
 int launch_rhs_tiled(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);

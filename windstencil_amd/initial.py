@@ -166,6 +166,26 @@ def blobs_with_shock(nx, ny, nz, seed, mach=2.0, x_shock=0.5, gamma=1.4, **kw):
     return rho, u, v, w, temperature(rho, p, gamma)
 
 
+def blast(nx, ny, nz, seed, ratio, radius=0.25, mach=0.5, noise=0.01, gamma=1.4):
+    """Spherical blast: pressure `ratio` times the ambient p = 1/gamma inside a sphere of
+    `radius` times the shortest box side around a seeded centre (periodic distance), the
+    same density (rho = 1) everywhere, a Mach-`mach` x-stream, and +-`noise`
+    multiplicative seeded noise on rho and p.  The jump of the split fluxes across the
+    sphere grows like ratio (momentum) to ratio^1.5 (energy)."""
+    rng = SplitMix64(seed)
+    x, y, z, L = _coords(nx, ny, nz)
+    c = (rng.uniform(0, L[0]), rng.uniform(0, L[1]), rng.uniform(0, L[2]))
+    r2 = (_periodic_delta(x, c[0], L[0]) ** 2 + _periodic_delta(y, c[1], L[1]) ** 2
+          + _periodic_delta(z, c[2], L[2]) ** 2)
+    inside = np.broadcast_to(r2 < (radius * min(L)) ** 2, (nz, ny, nx))
+    n = nx * ny * nz
+    rho = 1.0 + noise * rng.uniform_array(n).reshape(nz, ny, nx)
+    p = np.where(inside, ratio, 1.0) / gamma * (1.0 + noise * rng.uniform_array(n).reshape(nz, ny, nx))
+    u = np.full((nz, ny, nx), float(mach))
+    z0 = np.zeros((nz, ny, nx))
+    return rho, u, z0.copy(), z0.copy(), temperature(rho, p, gamma)
+
+
 def multiblock_draw(seed, nblocks=48, lo=32, hi=256):
     """Seeded block extents for the batched workload: each dimension U{lo..hi}."""
     rng = SplitMix64(seed)
