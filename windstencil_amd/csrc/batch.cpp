@@ -4,13 +4,16 @@
 // state.  The inviscid RHS of all blocks runs as one launch per direction over a device
 // tile list (block, tile coordinates) built here for every tile shape: one work-group
 // per tile, or a persistent grid walking contiguous runs of the list (the batch's
-// schedule); the ghost fill, sound speed and fused RK3 stage run per block.  A
+// schedule); the ghost fill, sound speed, fused RK3 stage and CFL step run as one launch
+// each over all blocks (a table of per-block work-group ranges), per block where that
+// table is not available (a first call under a stream capture).  A
 // batch step uses one dt for every block: the minimum of the blocks' CFL steps.  A
 // whole SSP-RK3 step (CFL reduction included) can be captured and replayed as one
 // hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -39,6 +42,13 @@ struct ws_batch {
     int ntiles[WS_TILE_COUNT][3] = {};
     double *d_dts = nullptr;    // per-block CFL steps
     double *d_dt = nullptr;     // the batch step (their minimum)
+    // the table of the batched helper kernels (built at the first use outside a capture;
+    // aux_second: its second field sets are the blocks' second primitive sets)
+    AuxBlock *d_aux = nullptr;
+    bool aux_second = false;
+    long ghost_wgs = 0, cell_wgs = 0;
+    double *d_cfl_part = nullptr;   // the CFL maxima of the batched stage kernel (cell_wgs)
+    bool cfl_in_batch = false;      // the last step's stage kernel wrote them there
     hipStream_t capture_stream = nullptr;
     long long cells = 0;
 };
@@ -130,6 +140,73 @@ bool prim_tables(ws_batch *b, hipStream_t s)
     return true;
 }
 
+// The table of the batched helper kernels (ghost fill, sound speed, stage, CFL step: one
+// launch each for every block), with the blocks' second primitive sets once prim_tables
+// has made them.  Built or refreshed outside a stream capture; false if it is not there.
+bool aux_table(ws_batch *b, hipStream_t s)
+{
+    const bool second = b->d_blocks2 != nullptr;
+    if (b->d_aux && b->aux_second == second) return true;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return b->d_aux != nullptr;
+    const size_t n = b->ctx.size();
+    std::vector<AuxBlock> t(n);
+    long gw = 0, cw = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const ws_state &st = b->st[i];
+        const BatchBlock &h = b->host_blocks[i];
+        const KParams &k = b->ctx[i]->kp;
+        AuxBlock &a = t[i];
+        std::memset(&a, 0, sizeof(a));
+        const ws_field s0[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
+        const ws_field s1[6] = {h.od, h.ou, h.ov, h.ow, h.oT, h.occ};
+        for (int q = 0; q < 6; ++q) {
+            a.g[0][q] = s0[q];
+            a.g[1][q] = second ? s1[q] : s0[q];
+        }
+        a.P = st.P;
+        a.fa = h.fa;
+        a.fb = h.fb;
+        a.n[0] = k.nx; a.n[1] = k.ny; a.n[2] = k.nz;
+        a.nyg = k.nyg;
+        a.hmin = std::fmin(k.h[0], std::fmin(k.h[1], k.h[2]));
+        const long ghosts = 2L * LAP * ((long)k.ny * k.nz + (long)k.nx * k.nz + (long)k.nx * k.ny);
+        const long cells = (long)k.nx * k.ny * k.nz;
+        a.ghost_wg0 = gw; gw += (ghosts + 255) / 256; a.ghost_wg1 = gw;
+        a.cell_wg0 = cw; cw += (cells + 255) / 256; a.cell_wg1 = cw;
+    }
+    if (gw > 0x7fffffffL || cw > 0x7fffffffL) return false;
+    if (!b->d_aux) {
+        if (hipMalloc(&b->d_aux, sizeof(AuxBlock) * n) != hipSuccess) { b->d_aux = nullptr; return false; }
+        if (hipMalloc(&b->d_cfl_part, sizeof(double) * cw) != hipSuccess) {
+            hipFree(b->d_aux);
+            b->d_aux = nullptr;
+            b->d_cfl_part = nullptr;
+            return false;
+        }
+    }
+    // a refresh overwrites a table earlier launches may still read
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        hipMemcpy(b->d_aux, t.data(), sizeof(AuxBlock) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return false;
+    b->aux_second = second;
+    b->ghost_wgs = gw;
+    b->cell_wgs = cw;
+    return true;
+}
+
+// ghost fill and sound speed of every block (the batched form when the table is there)
+int batch_prepare(ws_batch *b, hipStream_t s)
+{
+    if (aux_table(b, s))
+        return launch_prepare_batch(b->ctx[0]->kp, b->d_aux, (int)b->ctx.size(), b->ghost_wgs, b->cell_wgs, s);
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        int r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
+        if (r) return r;
+    }
+    return WS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -212,6 +289,8 @@ int ws_batch_destroy(ws_batch *b)
             if (b->d_tiles[shape][dir]) hipFree(b->d_tiles[shape][dir]);
     if (b->d_dts) hipFree(b->d_dts);
     if (b->d_dt) hipFree(b->d_dt);
+    if (b->d_aux) hipFree(b->d_aux);
+    if (b->d_cfl_part) hipFree(b->d_cfl_part);
     if (b->capture_stream) hipStreamDestroy(b->capture_stream);
     delete b;
     return WS_OK;
@@ -291,11 +370,8 @@ int ws_batch_inviscid_rhs(ws_batch *b, hipStream_t s)
 int ws_batch_du_comput(ws_batch *b, hipStream_t s)
 {
     if (!b) return set_error(WS_ERR_INVALID, "null batch");
-    for (size_t i = 0; i < b->ctx.size(); ++i) {
-        int r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
-        if (r) return r;
-    }
-    int r = batch_rhs(b, s);
+    int r = batch_prepare(b, s);
+    if (!r) r = batch_rhs(b, s);
     if (r || !b->base.viscous) return r;
     // viscous terms, block by block (memory-bound kernels; each block owns its scratch)
     for (size_t i = 0; i < b->ctx.size(); ++i) {
@@ -336,16 +412,27 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
     // primitives, stage 1 from the blocks' own fields into their second sets (d_blocks),
     // stage 2 back (d_blocks2); a ghost fill of the six written fields follows.
     const bool prims = fuse && b->ctx[0]->fuse_stage > 1 && prim_tables(b, s);
+    // the helper kernels of all blocks in one launch each (the table holds the second
+    // primitive sets when prims is on: prim_tables runs first and aux_table follows it)
+    const bool aux = aux_table(b, s) && (!prims || b->aux_second);
+    const int nb = (int)b->ctx.size();
+    const bool planes = b->base.bc_planes != 0;
+    b->cfl_in_batch = false;
     // the blocks are periodic (ws_batch_create); stages 1 and 2 leave the next stage's
     // inputs prepared (fused stage kernel writing the sound speed, then one ghost fill)
     for (int krk = 1; krk <= 3 && !r; ++krk) {
-        for (size_t i = 0; i < b->ctx.size() && !r && krk == 1; ++i) r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
+        if (krk == 1) {
+            if (aux) r = launch_prepare_batch(b->ctx[0]->kp, b->d_aux, nb, b->ghost_wgs, b->cell_wgs, s);
+            for (size_t i = 0; i < b->ctx.size() && !r && !aux; ++i) r = prepare_rhs_inputs(b->ctx[i], &b->st[i], s);
+        }
         BatchStage bs{dt_dev, b->base.dt, krk, swapped};
         if (prims && krk < 3) {
             bs.prims = 1;
-            bs.planes = b->base.bc_planes != 0;
+            bs.planes = planes;
             if (!r) r = batch_rhs(b, s, &bs, krk == 2 ? b->d_blocks2 : b->d_blocks);
-            for (size_t i = 0; i < b->ctx.size() && !r; ++i) {
+            // stage 1 wrote the second sets, stage 2 the state's fields
+            if (!r && aux) r = launch_ghost_fill_batch(b->d_aux, nb, b->ghost_wgs, krk == 1 ? 1 : 0, s);
+            for (size_t i = 0; i < b->ctx.size() && !r && !aux; ++i) {
                 const BatchBlock &h = b->host_blocks[i];
                 const ws_state &st = b->st[i];
                 const ws_field g1[6] = {h.od, h.ou, h.ov, h.ow, h.oT, h.occ};
@@ -358,6 +445,15 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
         for (size_t i = 0; i < b->ctx.size() && !r && b->base.viscous; ++i) {
             const ws_state &st = b->st[i];
             r = launch_viscous(*b->ctx[i], st.u, st.v, st.w, st.T, nullptr, st.du, s);
+        }
+        if (fuse && aux) {
+            // the last RHS launch applied the stage: the primitives of every block's f
+            const bool last = cfl_next && krk == 3;
+            if (!r) r = launch_stage_batch(b->ctx[0]->kp, b->d_aux, nb, b->cell_wgs, swapped, planes, krk < 3,
+                                           last ? b->d_cfl_part : nullptr, s);
+            if (!r && krk < 3) r = launch_ghost_fill_batch(b->d_aux, nb, b->ghost_wgs, 0, s);
+            if (!r && last) b->cfl_in_batch = true;
+            continue;
         }
         for (size_t i = 0; i < b->ctx.size() && !r; ++i) {
             ws_ctx *c = b->ctx[i];
@@ -389,6 +485,9 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
     for (ws_ctx *c : b->ctx) staged = staged && cfl_stage_buffer(c);
     auto cfl_dt = [&](bool first, hipStream_t ss) {
         if (first || !staged) return ws_batch_cfl_dt(b, cfl, dt_dev, ss);
+        // the last step's stage kernel left its maxima in the batch's buffer
+        if (b->cfl_in_batch)
+            return launch_cfl_batch(b->d_aux, (int)b->ctx.size(), b->d_cfl_part, cfl, b->d_dts, dt_dev, ss);
         for (size_t i = 0; i < b->ctx.size(); ++i) {
             ws_ctx *c = b->ctx[i];
             const int q = launch_cfl_final(*c, c->cfl_stage, (int)c->cfl_stage_n, cfl, b->d_dts + i, ss);

@@ -20,14 +20,12 @@ struct FieldList {
 // slabs of one field:
 // x slabs (2*LAP*ny*nz), y slabs (2*LAP*nx*nz), z slabs (2*LAP*nx*ny).
 // Ghost points read only interior points, so the three directions need no ordering.
-__global__ void __launch_bounds__(NT) k_ghost_fill(FieldList fl, int nx, int ny, int nz, int nyg, int px, int py,
-                                                   int pz)
+__device__ __forceinline__ void ghost_point(const ws_field &f, int nx, int ny, int nz, int nyg, bool px, bool py,
+                                            bool pz, long i)
 {
-    const ws_field f = fl.f[blockIdx.y];
     // slabs of non-periodic directions are left to the caller
     const long nxs = px ? 2L * LAP * ny * nz : 0, nys = py ? 2L * LAP * nx * nz : 0;
     const long nzs = pz ? 2L * LAP * nx * ny : 0;
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int dx, dy, dz, sx, sy, sz;
     if (i < nxs) {
         const int g = (int)(i % (2 * LAP));            // 0..3 left ghosts, 4..7 right ghosts
@@ -54,6 +52,13 @@ __global__ void __launch_bounds__(NT) k_ghost_fill(FieldList fl, int nx, int ny,
         return;
     }
     f.ptr[fidx(f.pitch, nyg, dx, dy, dz)] = f.ptr[fidx(f.pitch, nyg, sx, sy, sz)];
+}
+
+__global__ void __launch_bounds__(NT) k_ghost_fill(FieldList fl, int nx, int ny, int nz, int nyg, int px, int py,
+                                                   int pz)
+{
+    ghost_point(fl.f[blockIdx.y], nx, ny, nz, nyg, px != 0, py != 0, pz != 0,
+                (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(NT) k_sound_speed(ws_field T, ws_field cc, int sx, int sy, int sz, int nx,
@@ -211,6 +216,23 @@ __device__ __forceinline__ double stage_cell(const KParams &k, ws_soa f, ws_soa 
     return CFLP ? fmax(fabs(p.u) + c, fmax(fabs(p.v) + c, fabs(p.w) + c)) : 0.0;
 }
 
+// *out = the work-group's maximum of mx (every thread calls it); fmax is exact and ignores
+// NaN, so the order of the reduction does not matter
+__device__ __forceinline__ void wg_max(double mx, double *out)
+{
+    __shared__ double wmax[NT / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+#pragma unroll
+        for (int q = 1; q < NT / 64; ++q) b = fmax(b, wmax[q]);
+        *out = b;
+    }
+}
+
 template <int KRK, bool PLANES, bool SOUND, bool CFLP>
 __global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws_soa du, ws_soa fl, double dt_val,
                                               const double *dt_dev, ws_field d, ws_field u, ws_field v, ws_field w,
@@ -220,20 +242,7 @@ __global__ void __launch_bounds__(NT) k_stage(KParams k, ws_soa f, ws_soa fn, ws
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double mx = 0.0;
     if (i < total) mx = stage_cell<KRK, PLANES, SOUND, CFLP>(k, f, fn, du, fl, dt_val, dt_dev, d, u, v, w, T, P, cc, i);
-    if (CFLP) {
-        // fmax is exact and ignores NaN, so the order of the reduction does not matter
-        __shared__ double wmax[NT / 64];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
-        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double b = wmax[0];
-#pragma unroll
-            for (int q = 1; q < NT / 64; ++q) b = fmax(b, wmax[q]);
-            cfl_part[blockIdx.x] = b;
-        }
-    }
+    if (CFLP) wg_max(mx, cfl_part + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(NT) k_fill(double value, double *ptr, uint32_t pitch, uint32_t sx, uint32_t sy,
@@ -302,6 +311,84 @@ __global__ void __launch_bounds__(NT) k_min_reduce(const double *v, int n, doubl
         __syncthreads();
     }
     if (threadIdx.x == 0) *out = red[0];
+}
+
+// ---- batched helpers: every block of a batch in one launch ------------------------------
+// Each block owns a contiguous range of the grid's work-groups (AuxBlock); a work-group
+// finds its block by bisection over the ascending range starts (uniform, scalar loads) and
+// then does what the single-block kernel's work-group of the same index in that block does.
+
+template <bool GHOST>
+__device__ __forceinline__ const AuxBlock &aux_block(const AuxBlock *tab, int nb, long wg)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((GHOST ? tab[mid].ghost_wg0 : tab[mid].cell_wg0) <= wg) lo = mid;
+        else hi = mid - 1;
+    }
+    return tab[lo];
+}
+
+__global__ void __launch_bounds__(NT) k_ghost_fill_batch(const AuxBlock *tab, int nb, int set)
+{
+    const long wg = blockIdx.x;
+    const AuxBlock &b = aux_block<true>(tab, nb, wg);
+    ghost_point(b.g[set][blockIdx.y], b.n[0], b.n[1], b.n[2], b.nyg, true, true, true,
+                (wg - b.ghost_wg0) * NT + threadIdx.x);
+}
+
+// the sound speed of every block's interior (k_sound_flat's expression)
+__global__ void __launch_bounds__(NT) k_sound_batch(const AuxBlock *tab, int nb, double ama)
+{
+#pragma clang fp contract(off)
+    const long wg = blockIdx.x;
+    const AuxBlock &b = aux_block<false>(tab, nb, wg);
+    const int nx = b.n[0], ny = b.n[1];
+    const long i = (wg - b.cell_wg0) * NT + threadIdx.x;
+    if (i >= (long)nx * ny * b.n[2]) return;
+    const int x = (int)(i % nx) + LAP, y = (int)((i / nx) % ny) + LAP, z = (int)(i / ((long)nx * ny)) + LAP;
+    const ws_field &T = b.g[0][4], &cc = b.g[0][5];
+    cc.ptr[fidx(cc.pitch, b.nyg, x, y, z)] = sqrt(T.ptr[fidx(T.pitch, b.nyg, x, y, z)]) / ama;
+}
+
+// k_stage<0, PLANES, SOUND, CFLP> of every block: work-group cell_wg0 + q of a block covers
+// the block's cells [256 q, 256 q + 256), as work-group q of the single-block launch does,
+// so the CFL partials of a block are the single-block kernel's
+template <bool PLANES, bool SOUND, bool CFLP>
+__global__ void __launch_bounds__(NT) k_stage_batch(KParams k, const AuxBlock *tab, int nb, int swapped,
+                                                    double *cfl_part)
+{
+    const long wg = blockIdx.x;
+    const AuxBlock &b = aux_block<false>(tab, nb, wg);
+    KParams kb = k;
+    kb.nx = b.n[0]; kb.ny = b.n[1]; kb.nz = b.n[2]; kb.nyg = b.nyg;
+    const long total = (long)kb.nx * kb.ny * kb.nz;
+    const long i = (wg - b.cell_wg0) * NT + threadIdx.x;
+    const ws_soa f = swapped ? b.fb : b.fa;
+    double mx = 0.0;
+    if (i < total)
+        mx = stage_cell<0, PLANES, SOUND, CFLP>(kb, f, f, f, ws_soa{}, 0.0, nullptr, b.g[0][0], b.g[0][1], b.g[0][2],
+                                                b.g[0][3], b.g[0][4], b.P, b.g[0][5], i);
+    if (CFLP) wg_max(mx, cfl_part + wg);
+}
+
+// one work-group per block: k_cfl_final over the block's partials
+__global__ void __launch_bounds__(NT) k_cfl_final_batch(const AuxBlock *tab, const double *part, double cfl,
+                                                        double *dts)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[NT];
+    const AuxBlock &b = tab[blockIdx.x];
+    double mx = 0.0;
+    for (long i = b.cell_wg0 + threadIdx.x; i < b.cell_wg1; i += NT) mx = fmax(mx, part[i]);
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dts[blockIdx.x] = cfl * b.hmin / red[0];
 }
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + NT - 1) / NT); }
@@ -471,6 +558,56 @@ int launch_min_reduce(const double *v, int n, double *out, hipStream_t s)
     hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(NT), 0, s, v, n, out);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "min reduce launch");
+}
+
+int launch_ghost_fill_batch(const AuxBlock *tab, int nb, long ghost_wgs, int set, hipStream_t s)
+{
+    if (set < 0 || set > 1) return set_error(WS_ERR_INVALID, "field set must be 0 or 1");
+    if (ghost_wgs <= 0 || nb <= 0) return WS_OK;
+    hipLaunchKernelGGL(k_ghost_fill_batch, dim3((unsigned)ghost_wgs, 6), dim3(NT), 0, s, tab, nb, set);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "batched ghost fill launch");
+}
+
+int launch_prepare_batch(const KParams &k, const AuxBlock *tab, int nb, long ghost_wgs, long cell_wgs,
+                         hipStream_t s)
+{
+    if (nb <= 0 || cell_wgs <= 0) return WS_OK;
+    hipLaunchKernelGGL(k_sound_batch, dim3((unsigned)cell_wgs), dim3(NT), 0, s, tab, nb, k.ama);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "batched sound speed launch");
+    return launch_ghost_fill_batch(tab, nb, ghost_wgs, 0, s);
+}
+
+int launch_stage_batch(const KParams &k, const AuxBlock *tab, int nb, long cell_wgs, int swapped, bool planes,
+                       bool sound, double *cfl_part, hipStream_t s)
+{
+    if (nb <= 0 || cell_wgs <= 0) return WS_OK;
+    const dim3 g((unsigned)cell_wgs);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, dim3(NT), 0, s, k, tab, nb, swapped, cfl_part); };
+    const int sel = (planes ? 4 : 0) + (sound ? 2 : 0) + (cfl_part ? 1 : 0);
+    switch (sel) {
+    case 0: go(k_stage_batch<false, false, false>); break;
+    case 1: go(k_stage_batch<false, false, true>); break;
+    case 2: go(k_stage_batch<false, true, false>); break;
+    case 3: go(k_stage_batch<false, true, true>); break;
+    case 4: go(k_stage_batch<true, false, false>); break;
+    case 5: go(k_stage_batch<true, false, true>); break;
+    case 6: go(k_stage_batch<true, true, false>); break;
+    default: go(k_stage_batch<true, true, true>); break;
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "batched stage launch");
+}
+
+int launch_cfl_batch(const AuxBlock *tab, int nb, const double *cfl_part, double cfl, double *dts, double *dt_dev,
+                     hipStream_t s)
+{
+    if (nb <= 0) return set_error(WS_ERR_INVALID, "empty batch");
+    hipLaunchKernelGGL(k_cfl_final_batch, dim3(nb), dim3(NT), 0, s, tab, cfl_part, cfl, dts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "batched cfl launch");
+    return launch_min_reduce(dts, nb, dt_dev, s);
 }
 
 }  // namespace ws

@@ -62,10 +62,13 @@ struct Tile {
     static_assert(5 * W * F <= 10 * QS, "face buffer must fit in the split buffer");
     // LDS index of point p (0..NL-1) of line l (0..W-1)
     __device__ static int pidx(int l, int p) { return l * LS + p; }
-    // face buffer index of face f of line l (line-major; F = L + 1 is odd for the even
-    // tile lengths used, so the y/z flux difference, which reads one face of
-    // consecutive lines per lane, touches distinct banks)
-    __device__ static int fidx(int l, int f) { return l * F + f; }
+    // face buffer index of face f of line l.  The flux difference reads, per lane, the two
+    // faces of one cell, with consecutive lanes on consecutive cells of a line in x tiles
+    // and on consecutive lines in y/z tiles (so that their du rows stay coalesced): x
+    // tiles store faces line-major, y/z tiles face-major, and either way the lanes of
+    // one read touch consecutive doubles (a line-major y/z buffer with F = L + 1 put the
+    // lanes (l, i) and (l + 1, i - 1) on one bank).
+    __device__ static int fidx(int l, int f) { return DIR == 0 ? l * F + f : f * W + l; }
 };
 
 // Where a tile lies: x0 always tiles x; the line axis is x (DIR 0), y (DIR 1) or z
@@ -282,18 +285,18 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                     for (int j = 0; j < K; ++j)
                         closed(j, [&] { return bP[j] + (double)cP[j]; }, [&] { return bM[j] + (double)cM[j]; });
             } else {
-                double nP[K], dP[K], nM[K], dM[K];
-                weno_k<false, K>([&](int j) { return qp[j]; }, nP, dP);
-                weno_k<true, K>([&](int j) { return qm[j]; }, nM, dM);
+                double nP[K], dP[K], nM[K], dM[K], bP[K], bM[K];
+                weno_k<false, K>([&](int j) { return qp[j]; }, nP, dP, bP);
+                weno_k<true, K>([&](int j) { return qm[j]; }, nM, dM, bM);
 #pragma unroll
                 for (int j = 0; j < K; ++j) {
                     const double den = dP[j] * dM[j];
-                    Fv[m][j] = fma(nP[j], dM[j], nM[j] * dP[j]) * recip_nr(den);
+                    Fv[m][j] = (bP[j] + bM[j]) + fma(nP[j], dM[j], nM[j] * dP[j]) * recip_nr(den);
                 }
                 if (near)
 #pragma unroll
                     for (int j = 0; j < K; ++j)
-                        closed(j, [&] { return nP[j] / dP[j]; }, [&] { return nM[j] / dM[j]; });
+                        closed(j, [&] { return bP[j] + nP[j] / dP[j]; }, [&] { return bM[j] + nM[j] / dM[j]; });
             }
         }
         // Faces whose FP32 weights left FP32's range (strong discontinuities of the split
@@ -308,19 +311,19 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                 for (int j = 0; j < K; ++j) {
                     if (!((bad_mix >> (m * K + j)) & 1u)) continue;
                     const double *qp = sp + m * TL::QS + base, *qm = sp + (5 + m) * TL::QS + base;
-                    double nP, dP, nM, dM, val;
+                    double nP, dP, nM, dM, bP, bM, val;
                     const int fg = gl0 + f0 + j;
                     if (BND && ((lo && fg <= 3) || (hi && fg >= n_dom - 3))) {
                         auto cellP = [&](int cg) { return sp[m * TL::QS + TL::pidx(line, cg - gl0 + LAP)]; };
                         auto cellM = [&](int cg) { return sp[(5 + m) * TL::QS + TL::pidx(line, cg - gl0 + LAP)]; };
                         double pv, mv;
-                        if (!closure_p(cellP, n_dom, fg, lo, hi, pv)) { face64<false>(qp + j, nP, dP); pv = nP / dP; }
-                        if (!closure_m(cellM, n_dom, fg, lo, hi, mv)) { face64<true>(qm + j, nM, dM); mv = nM / dM; }
+                        if (!closure_p(cellP, n_dom, fg, lo, hi, pv)) { face64<false>(qp + j, nP, dP, bP); pv = bP + nP / dP; }
+                        if (!closure_m(cellM, n_dom, fg, lo, hi, mv)) { face64<true>(qm + j, nM, dM, bM); mv = bM + nM / dM; }
                         val = pv + mv;
                     } else {
-                        face64<false>(qp + j, nP, dP);
-                        face64<true>(qm + j, nM, dM);
-                        val = fma(nP, dM, nM * dP) * recip_nr(dP * dM);
+                        face64<false>(qp + j, nP, dP, bP);
+                        face64<true>(qm + j, nM, dM, bM);
+                        val = (bP + bM) + fma(nP, dM, nM * dP) * recip_nr(dP * dM);
                     }
 #pragma unroll
                     for (int q = 0; q < 5; ++q)

@@ -223,17 +223,17 @@ __device__ __forceinline__ double sqrt_nr(double x)
 // squared third differences (with eps added) are formed once per point and shared by the
 // K faces.  The linear weights and the 1/12 of the candidates are folded into the
 // candidate coefficients, and the weighted sums are evaluated in Horner form over the
-// prefix products of (eps + S_k).  Face j's value is num[j] / den[j].
+// prefix products of (eps + S_k).  The candidates are taken relative to the face's base
+// point w3 and written in Newton form over the shared forward differences, so they cost
+// three FMAs each; face j's value is base[j] + num[j] / den[j].
 template <bool REV, int K, typename Load>
-__device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[K])
+__device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[K], double (&base)[K])
 {
+    // Every fused multiply-add here is written out: implicit contraction could fuse a shared
+    // difference differently at different positions j, and a face computed by two threads
+    // (at a tile edge) must come out bit for bit the same in both.
+#pragma clang fp contract(off)
     constexpr int NP = K + 7;
-    // d_k / 12 times the candidate coefficients (candidate k on window points k..k+3)
-    constexpr double q00 = kD0 * (-3.0 / 12.0), q01 = kD0 * (13.0 / 12.0), q02 = kD0 * (-23.0 / 12.0), q03 = kD0 * (25.0 / 12.0);
-    constexpr double q11 = kD1 * (1.0 / 12.0), q12 = kD1 * (-5.0 / 12.0), q13 = kD1 * (13.0 / 12.0), q14 = kD1 * (3.0 / 12.0);
-    constexpr double q22 = kD2 * (-1.0 / 12.0), q23 = kD2 * (7.0 / 12.0), q24 = kD2 * (7.0 / 12.0), q25 = kD2 * (-1.0 / 12.0);
-    constexpr double q33 = kD3 * (3.0 / 12.0), q34 = kD3 * (13.0 / 12.0), q35 = kD3 * (-5.0 / 12.0), q36 = kD3 * (1.0 / 12.0);
-    constexpr double q44 = kD4 * (25.0 / 12.0), q45 = kD4 * (-23.0 / 12.0), q46 = kD4 * (13.0 / 12.0), q47 = kD4 * (-3.0 / 12.0);
     double s[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) s[i] = ld(i);
@@ -271,11 +271,20 @@ __device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[
         const double e3 = fma(a3, a3, fma(b3, b3, C3[i3(3)]));
         double e4 = fma(a4, a4, fma(b4, b4, C3[i3(4)]));
         e4 = fmax(fmax(fmax(e0, e1), fmax(e2, e3)), e4);
-        const double c0 = fma(q03, w(3), fma(q02, w(2), fma(q01, w(1), q00 * w(0))));
-        const double c1 = fma(q14, w(4), fma(q13, w(3), fma(q12, w(2), q11 * w(1))));
-        const double c2 = fma(q25, w(5), fma(q24, w(4), fma(q23, w(3), q22 * w(2))));
-        const double c3 = fma(q36, w(6), fma(q35, w(5), fma(q34, w(4), q33 * w(3))));
-        const double c4 = fma(q47, w(7), fma(q46, w(6), fma(q45, w(5), q44 * w(4))));
+        // candidates relative to the base point w(3), in Newton form over the forward
+        // differences at each candidate's first point (linear weights folded in): each
+        // candidate's coefficients sum to 12, so the base drops out
+        const double sd = REV ? -1.0 : 1.0;   // sign of odd differences of the reversed window
+        auto d1 = [&](int k) { return D1[REV ? j + 6 - k : j + k]; };
+        auto d2 = [&](int k) { return D2[REV ? j + 5 - k : j + k]; };
+        auto d3 = [&](int k) { return D3[REV ? j + 4 - k : j + k]; };
+        const double c0 = fma(sd * kD0 * (13.0 / 12.0), d3(0), fma(kD0 * (4.0 / 3.0), d2(0), (sd * kD0 * 0.5) * d1(0)));
+        const double c1 = fma(sd * kD1 * (1.0 / 4.0), d3(1), fma(kD1 * (5.0 / 6.0), d2(1), (sd * kD1 * 0.5) * d1(1)));
+        const double c2 = fma(sd * kD2 * (-1.0 / 12.0), d3(2), fma(kD2 * (1.0 / 3.0), d2(2), (sd * kD2 * 0.5) * d1(2)));
+        const double c3 = fma(sd * kD3 * (1.0 / 12.0), d3(3), fma(kD3 * (-1.0 / 6.0), d2(3), (sd * kD3 * 0.5) * d1(3)));
+        const double c4 = fma(sd * kD4 * (-1.0 / 4.0), d3(4), fma(kD4 * (1.0 / 3.0), d2(4),
+                              fma(sd * kD4 * -0.5, d1(4), (sd * kD4) * d1(3))));
+        base[j] = w(3);
         const double p01 = e0 * e1;
         double N = fma(c0, e1, c1 * e0);
         double D = fma(kD0, e1, kD1 * e0);
@@ -290,14 +299,15 @@ __device__ __forceinline__ void weno_k(Load ld, double (&num)[K], double (&den)[
     }
 }
 
-// One face of weno_k from the 8-point window q[0..7]: num / den is the face value.
+// One face of weno_k from the 8-point window q[0..7]: base + num / den is the face value.
 template <bool REV>
-__device__ __forceinline__ void face64(const double *q, double &num, double &den)
+__device__ __forceinline__ void face64(const double *q, double &num, double &den, double &base)
 {
-    double n[1], d[1];
-    weno_k<REV, 1>([&](int i) { return q[i]; }, n, d);
+    double n[1], d[1], b[1];
+    weno_k<REV, 1>([&](int i) { return q[i]; }, n, d, b);
     num = n[0];
     den = d[0];
+    base = b[0];
 }
 
 // Mixed-precision form of weno_k: the window stays FP64 in LDS; its first differences
@@ -320,10 +330,17 @@ __device__ __forceinline__ void face64(const double *q, double &num, double &den
 template <bool REV, int K, typename Load>
 __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (&corr)[K], unsigned &bad)
 {
+    // explicit FMAs only, as in weno_k: the same face must round the same at every position j
+#pragma clang fp contract(off)
     bad = 0;
     constexpr int NP = K + 7;
     constexpr float fD0 = (float)kD0, fD1 = (float)kD1, fD2 = (float)kD2, fD3 = (float)kD3, fD4 = (float)kD4;
-    constexpr float fB = 39.0f, fC = 39.05f, fEps = 3.6e-9f;
+    // second and third differences carry sqrt(39): the 780-term of the indicators (39 in units
+    // of the 720-term) becomes a plain square, and the other terms take 1/sqrt(39) in their
+    // constant coefficients
+    constexpr float fR = 6.2449979983983983f;          // sqrt(39)
+    constexpr float fIR = 0.16012815380508713f;        // 1/sqrt(39)
+    constexpr float fC = 39.05f / 39.0f, fEps = 3.6e-9f;
     double s[NP];
 #pragma unroll
     for (int i = 0; i < NP; ++i) s[i] = ld(i);
@@ -331,7 +348,7 @@ __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (
 #pragma unroll
     for (int i = 0; i < NP - 1; ++i) { D1[i] = (float)(s[i + 1] - s[i]); G1[i] = 6.0f * D1[i]; }
 #pragma unroll
-    for (int i = 0; i < NP - 2; ++i) { D2[i] = D1[i + 1] - D1[i]; B2[i] = (fB * D2[i]) * D2[i]; }
+    for (int i = 0; i < NP - 2; ++i) { D2[i] = (D1[i + 1] - D1[i]) * fR; B2[i] = D2[i] * D2[i]; }
 #pragma unroll
     for (int i = 0; i < NP - 3; ++i) { D3[i] = D2[i + 1] - D2[i]; C3[i] = fmaf(fC * D3[i], D3[i], fEps); }
 #pragma unroll
@@ -339,37 +356,37 @@ __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (
         auto i1 = [&](int k) { return REV ? j + 6 - k : j + k; };
         auto i2 = [&](int k) { return REV ? j + 5 - k : j + k; };
         auto i3 = [&](int k) { return REV ? j + 4 - k : j + k; };
-        const float sb = REV ? -1.0f : 1.0f;
-        const float a0 = fmaf(11.0f, D3[i3(0)], fmaf(sb * 15.0f, D2[i2(0)], G1[i1(0)]));
-        const float a1 = fmaf(2.0f, D3[i3(1)], fmaf(sb * 9.0f, D2[i2(1)], G1[i1(1)]));
-        const float a2 = fmaf(-1.0f, D3[i3(2)], fmaf(sb * 3.0f, D2[i2(2)], G1[i1(2)]));
-        const float a3 = fmaf(2.0f, D3[i3(3)], fmaf(sb * -3.0f, D2[i2(3)], G1[i1(3)]));
-        const float a4 = fmaf(11.0f, D3[i3(4)], fmaf(sb * -9.0f, D2[i2(4)], G1[i1(4)]));
+        const float sb = REV ? -fIR : fIR;
+        const float a0 = fmaf(11.0f * fIR, D3[i3(0)], fmaf(sb * 15.0f, D2[i2(0)], G1[i1(0)]));
+        const float a1 = fmaf(2.0f * fIR, D3[i3(1)], fmaf(sb * 9.0f, D2[i2(1)], G1[i1(1)]));
+        const float a2 = fmaf(-1.0f * fIR, D3[i3(2)], fmaf(sb * 3.0f, D2[i2(2)], G1[i1(2)]));
+        const float a3 = fmaf(2.0f * fIR, D3[i3(3)], fmaf(sb * -3.0f, D2[i2(3)], G1[i1(3)]));
+        const float a4 = fmaf(11.0f * fIR, D3[i3(4)], fmaf(sb * -9.0f, D2[i2(4)], G1[i1(4)]));
         const float b0 = REV ? fmaf(-2.0f, D3[j + 4], D2[j + 5]) : fmaf(2.0f, D3[j], D2[j]);
         const float b3 = REV ? D2[j + 2] + D3[j + 1] : D2[j + 3] - D3[j + 3];
         const float b4 = REV ? fmaf(2.0f, D3[j], D2[j + 1]) : fmaf(-2.0f, D3[j + 4], D2[j + 4]);
         const float bb1 = REV ? B2[j + 3] : B2[j + 2];
         const float bb2 = REV ? B2[j + 2] : B2[j + 3];
         // e_k = eps + S_k (eps rides in C3)
-        const float e0 = fmaf(a0, a0, fmaf(fB * b0, b0, C3[i3(0)]));
+        const float e0 = fmaf(a0, a0, fmaf(b0, b0, C3[i3(0)]));
         const float e1 = fmaf(a1, a1, bb1 + C3[i3(1)]);
         const float e2 = fmaf(a2, a2, bb2 + C3[i3(2)]);
-        const float e3 = fmaf(a3, a3, fmaf(fB * b3, b3, C3[i3(3)]));
-        float e4 = fmaf(a4, a4, fmaf(fB * b4, b4, C3[i3(4)]));
+        const float e3 = fmaf(a3, a3, fmaf(b3, b3, C3[i3(3)]));
+        float e4 = fmaf(a4, a4, fmaf(b4, b4, C3[i3(4)]));
         e4 = fmaxf(fmaxf(fmaxf(e0, e1), fmaxf(e2, e3)), e4);
-        // window relative to its base point w3: dw(k) = w(k) - w(3), from the FP32 differences
-        // (P: w(k) = s[j+k], so w(k+1) - w(k) = D1[j+k]; M: w(k) = s[j+7-k], w(k+1) - w(k) = -D1[j+6-k])
-        auto dw1 = [&](int k) { return REV ? -D1[j + 6 - k] : D1[j + k]; };  // w(k+1) - w(k)
-        const float r4 = dw1(3), r5 = r4 + dw1(4), r6 = r5 + dw1(5), r7 = r6 + dw1(6);
-        const float r2 = -dw1(2), r1 = r2 - dw1(1), r0 = r1 - dw1(0);
-        // candidates on the relative window (coefficients of each candidate sum to 12, so the
-        // base drops out), linear weights and 1/12 folded in
-        const float c0 = fmaf(fD0 * (13.0f / 12.0f), r1, fmaf(fD0 * (-23.0f / 12.0f), r2, fD0 * (-3.0f / 12.0f) * r0));
-        const float c1 = fmaf(fD1 * (3.0f / 12.0f), r4, fmaf(fD1 * (-5.0f / 12.0f), r2, fD1 * (1.0f / 12.0f) * r1));
-        const float c2 = fmaf(fD2 * (7.0f / 12.0f), r4, fmaf(fD2 * (-1.0f / 12.0f), r5, fD2 * (-1.0f / 12.0f) * r2));
-        const float c3 = fmaf(fD3 * (13.0f / 12.0f), r4, fmaf(fD3 * (-5.0f / 12.0f), r5, fD3 * (1.0f / 12.0f) * r6));
-        const float c4 = fmaf(fD4 * (25.0f / 12.0f), r4, fmaf(fD4 * (-23.0f / 12.0f), r5,
-                              fmaf(fD4 * (13.0f / 12.0f), r6, fD4 * (-3.0f / 12.0f) * r7)));
+        // candidates relative to the base point w3 in Newton form over the forward
+        // differences at each candidate's first point (D2 and D3 carry sqrt(39)); linear
+        // weights folded in
+        const float sd = REV ? -1.0f : 1.0f;
+        auto d1 = [&](int k) { return D1[REV ? j + 6 - k : j + k]; };
+        auto d2 = [&](int k) { return D2[REV ? j + 5 - k : j + k]; };
+        auto d3 = [&](int k) { return D3[REV ? j + 4 - k : j + k]; };
+        const float c0 = fmaf(sd * fD0 * (13.0f / 12.0f) * fIR, d3(0), fmaf(fD0 * (4.0f / 3.0f) * fIR, d2(0), (sd * fD0 * 0.5f) * d1(0)));
+        const float c1 = fmaf(sd * fD1 * (1.0f / 4.0f) * fIR, d3(1), fmaf(fD1 * (5.0f / 6.0f) * fIR, d2(1), (sd * fD1 * 0.5f) * d1(1)));
+        const float c2 = fmaf(sd * fD2 * (-1.0f / 12.0f) * fIR, d3(2), fmaf(fD2 * (1.0f / 3.0f) * fIR, d2(2), (sd * fD2 * 0.5f) * d1(2)));
+        const float c3 = fmaf(sd * fD3 * (1.0f / 12.0f) * fIR, d3(3), fmaf(fD3 * (-1.0f / 6.0f) * fIR, d2(3), (sd * fD3 * 0.5f) * d1(3)));
+        const float c4 = fmaf(sd * fD4 * (-1.0f / 4.0f) * fIR, d3(4), fmaf(fD4 * (1.0f / 3.0f) * fIR, d2(4),
+                              fmaf(sd * fD4 * -0.5f, d1(4), (sd * fD4) * d1(3))));
         const float p01 = e0 * e1;
         float N = fmaf(c0, e1, c1 * e0);
         float D = fmaf(fD0, e1, fD1 * e0);

@@ -129,6 +129,41 @@ struct BatchTiles {
     const BatchStage *stage;   // non-null: the z launch applies this RK3 stage (grid schedule)
 };
 
+// One block of a batch for the batched helper kernels (ghost fill, sound speed, the stage
+// kernel and the CFL step over every block in one launch).  A launch gives each block a
+// contiguous range of work-groups: [ghost_wg0, ghost_wg1) in the ghost-fill grid, one
+// element of a face slab per thread, and [cell_wg0, cell_wg1) in the cell grid, 256
+// interior cells per work-group in the order of the single-block kernels, so that the
+// blocks' work-groups see the cells they see there.
+struct AuxBlock {
+    ws_field g[2][6];    // fields of a ghost fill: the state's d, u, v, w, T, cc; the second set
+    ws_field P;          // the state's pressure (null: not kept)
+    ws_soa fa, fb;       // the block's f and fn as created
+    int n[3];
+    int nyg;
+    double hmin;         // min(hx, hy, hz)
+    long ghost_wg0, ghost_wg1;
+    long cell_wg0, cell_wg1;
+};
+
+// Batched helper launches over a device table of nb blocks (k_aux.hip).  ghost_wgs and
+// cell_wgs: the table's last ghost_wg1 and cell_wg1.
+// Periodic ghost fill of field set `set` (0 or 1) of every block.
+int launch_ghost_fill_batch(const AuxBlock *tab, int nb, long ghost_wgs, int set, hipStream_t s);
+// cc = sqrt(T) / Ma on every block's interior (set 0), then the ghost fill of set 0: the
+// inputs of a batched RHS, as prepare_rhs_inputs leaves them on the cells the RHS reads.
+int launch_prepare_batch(const KParams &k, const AuxBlock *tab, int nb, long ghost_wgs, long cell_wgs,
+                         hipStream_t s);
+// The stage kernel with krk 0 over every block (the last RHS launch already applied the
+// stage to f = fb when swapped, else fa): primitives of set 0 and P, with planes the
+// boundary planes, with sound the interior sound speed; cfl_part (cell_wgs entries): the
+// CFL maximum of each work-group.
+int launch_stage_batch(const KParams &k, const AuxBlock *tab, int nb, long cell_wgs, int swapped, bool planes,
+                       bool sound, double *cfl_part, hipStream_t s);
+// dts[b] = cfl * hmin / (maximum of block b's CFL partials), then *dt_dev = min over b.
+int launch_cfl_batch(const AuxBlock *tab, int nb, const double *cfl_part, double cfl, double *dts, double *dt_dev,
+                     hipStream_t s);
+
 // (tile shape, schedule, direction order) the autotuner found best for these extents and
 // precision; false if none
 bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order);
