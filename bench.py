@@ -85,6 +85,11 @@ ALGO_FLOPS_PER_CELL_SHARED_EPS = 6981
 # two peaks: 2477 / (177 / 78.6 + 2300 / 157.3) = 146.8 TFLOP/s of algorithmic FLOPs.
 ALGO_FLOPS_FP64_PART = 137 + 40
 ALGO_FLOPS_FP32_PART = 10 * 230
+# Algorithmic FLOPs of the viscous terms per cell (CD6), the same convention, counted by the
+# same op-counting build: the viscosity 6, twelve gradients 12 x 9, the stress, heat flux and
+# four fluxes formed for each direction 3 x 125, twelve flux derivatives added to du
+# 12 x (9 + 2).
+VISCOUS_FLOPS_PER_CELL_CD6 = 621
 
 
 def peak_tflops(mixed):
@@ -529,7 +534,7 @@ def main():
     viscous = None
     if not args.no_viscous:
         # the Navier-Stokes RHS: the same inviscid RHS plus the viscous terms (CD6,
-        # Re 1000), timed on the same state; the viscous kernels are HBM-bound
+        # Re 1000), timed on the same state
         vctx = W.Context(W.default_config(nx, ny, nz, viscous=6))
         W.check(L.ws_ctx_set_tile_shape(vctx.handle, int(shape.split("/")[0])))
         W.check(L.ws_ctx_set_schedule(vctx.handle, int(shape.split("/")[1])))
@@ -558,13 +563,20 @@ def main():
         torch.cuda.synchronize()
         vis_ms = statistics.mean(e[0].elapsed_time(e[1]) for e in evs)
         vis_bytes = VISCOUS_BYTES_PER_CELL * N
+        vis_tf = VISCOUS_FLOPS_PER_CELL_CD6 * N / (vis_ms * 1e-3) / 1e12
+        vis_gbs = vis_bytes / (vis_ms * 1e-3) / 1e9
+        # the fused kernel is bound by neither roof: it issues its own instruction stream (the
+        # flux points of a ring around each tile and of the z-chunk halo are computed twice)
+        # at two wavefronts per SIMD; both fractions are reported
         viscous = {"workload": "inviscid + viscous (CD6, Re 1000, Sutherland) RHS, FP64; viscous terms in one "
                                "fused kernel (k_vis_fused)",
                    "value": round(N / (ns_ms * 1e-3) / 1e6, 2), "unit": "Mcell-updates/s",
                    "ms_per_step": round(ns_ms, 4), "viscous_kernels_ms": round(vis_ms, 4),
-                   "viscous_roofline": {"bound": "hbm", "achieved": round(vis_bytes / (vis_ms * 1e-3) / 1e9, 1),
-                                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                        "frac": round(vis_bytes / (vis_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                   "viscous_roofline": {"bound": "valu", "achieved": round(vis_tf, 3), "peak": PEAK_FP64_TFLOPS,
+                                        "unit": "TFLOP/s", "frac": round(vis_tf / PEAK_FP64_TFLOPS, 4),
+                                        "algo_flops_per_cell": VISCOUS_FLOPS_PER_CELL_CD6,
+                                        "hbm_achieved_gbs": round(vis_gbs, 1),
+                                        "hbm_frac": round(vis_gbs / PEAK_HBM_GBS, 4),
                                         "bytes_per_cell": VISCOUS_BYTES_PER_CELL}}
         del vctx
 

@@ -556,12 +556,12 @@ void orc_rk3_run(const orc_params *p, int nsteps, double dt,
 
 double orc_cd6(const double *s)
 {
-    return (45.0 * (s[4] - s[2]) - 9.0 * (s[5] - s[1]) + (s[6] - s[0])) / 60.0;
+    return orc_cell_cd6(s);
 }
 
 double orc_cd8(const double *s)
 {
-    return (672.0 * (s[5] - s[3]) - 168.0 * (s[6] - s[2]) + 32.0 * (s[7] - s[1]) - 3.0 * (s[8] - s[0])) / 840.0;
+    return orc_cell_cd8(s);
 }
 
 void orc_amu(const orc_params *p, double re, double ref_t, const double *T, double *amu)
@@ -572,8 +572,7 @@ void orc_amu(const orc_params *p, double re, double ref_t, const double *T, doub
     for (k = 0; k < p->nz; k++)
         for (j = 0; j < p->ny; j++)
             for (i = 0; i < p->nx; i++) {
-                const double t = T[G(p, i, j, k)];
-                amu[S(p, i, j, k, 0)] = c0 * sqrt(t * t * t) / (tsb + t);
+                amu[S(p, i, j, k, 0)] = orc_cell_amu(T[G(p, i, j, k)], c0, tsb);
             }
 }
 
@@ -633,9 +632,6 @@ void orc_viscous_rhs(const orc_params *p, double re, double pr, double ref_t, in
     double *amu = (double *)malloc(sizeof(double) * N);
     double *ev = (double *)calloc(4 * NG, sizeof(double));
     const double *fld[4] = {u, v, w, T};
-    const double *Akx = metrics[0], *Aky = metrics[1], *Akz = metrics[2];
-    const double *Aix = metrics[3], *Aiy = metrics[4], *Aiz = metrics[5];
-    const double *Asx = metrics[6], *Asy = metrics[7], *Asz = metrics[8];
     int dir, var, m;
     long i, j, k;
     set_threads(p);
@@ -649,49 +645,17 @@ void orc_viscous_rhs(const orc_params *p, double re, double pr, double ref_t, in
                     for (i = 0; i < p->nx; i++) out[S(p, i, j, k, 0)] = cd_at(p, order, dir, fld[var], i, j, k);
         }
     for (dir = 0; dir < 3; dir++) {
-        const double *Ax = metrics[3 * dir], *Ay = metrics[3 * dir + 1], *Az = metrics[3 * dir + 2];
 #pragma omp parallel for private(i, j) schedule(static)
         for (k = 0; k < p->nz; k++)
             for (j = 0; j < p->ny; j++)
                 for (i = 0; i < p->nx; i++) {
                     const size_t c = S(p, i, j, k, 0), gg = G(p, i, j, k);
-                    const double uk = g[0 * N + c], vk = g[1 * N + c], wk = g[2 * N + c], Tk = g[3 * N + c];
-                    const double ui = g[4 * N + c], vi = g[5 * N + c], wi = g[6 * N + c], Ti = g[7 * N + c];
-                    const double us = g[8 * N + c], vs = g[9 * N + c], ws = g[10 * N + c], Ts = g[11 * N + c];
-                    double ux, vx, wx, uy, vy, wy, uz, vz, wz, div, a1;
-                    double s11, s12, s13, s22, s23, s33, amuk, Tx, Ty, Tz, E1, E2, E3, aj1, akx, aky, akz;
-                    ux = uk * Akx[gg] + ui * Aix[gg] + us * Asx[gg];
-                    vx = vk * Akx[gg] + vi * Aix[gg] + vs * Asx[gg];
-                    wx = wk * Akx[gg] + wi * Aix[gg] + ws * Asx[gg];
-                    uy = uk * Aky[gg] + ui * Aiy[gg] + us * Asy[gg];
-                    vy = vk * Aky[gg] + vi * Aiy[gg] + vs * Asy[gg];
-                    wy = wk * Aky[gg] + wi * Aiy[gg] + ws * Asy[gg];
-                    uz = uk * Akz[gg] + ui * Aiz[gg] + us * Asz[gg];
-                    vz = vk * Akz[gg] + vi * Aiz[gg] + vs * Asz[gg];
-                    wz = wk * Akz[gg] + wi * Aiz[gg] + ws * Asz[gg];
-                    div = ux + vy + wz;
-                    a1 = amu[c];
-                    s11 = (2.0 * ux - 2.0 / 3.0 * div) * a1;
-                    s22 = (2.0 * vy - 2.0 / 3.0 * div) * a1;
-                    s33 = (2.0 * wz - 2.0 / 3.0 * div) * a1;
-                    s12 = (uy + vx) * a1;
-                    s13 = (uz + wx) * a1;
-                    s23 = (vz + wy) * a1;
-                    amuk = amu[c] * vis_c;
-                    Tx = Tk * Akx[gg] + Ti * Aix[gg] + Ts * Asx[gg];
-                    Ty = Tk * Aky[gg] + Ti * Aiy[gg] + Ts * Asy[gg];
-                    Tz = Tk * Akz[gg] + Ti * Aiz[gg] + Ts * Asz[gg];
-                    E1 = u[gg] * s11 + v[gg] * s12 + w[gg] * s13 + amuk * Tx;
-                    E2 = u[gg] * s12 + v[gg] * s22 + w[gg] * s23 + amuk * Ty;
-                    E3 = u[gg] * s13 + v[gg] * s23 + w[gg] * s33 + amuk * Tz;
-                    aj1 = ajac[gg];
-                    akx = Ax[gg] * aj1;
-                    aky = Ay[gg] * aj1;
-                    akz = Az[gg] * aj1;
-                    ev[0 * NG + gg] = akx * s11 + aky * s12 + akz * s13;
-                    ev[1 * NG + gg] = akx * s12 + aky * s22 + akz * s23;
-                    ev[2 * NG + gg] = akx * s13 + aky * s23 + akz * s33;
-                    ev[3 * NG + gg] = akx * E1 + aky * E2 + akz * E3;
+                    double gv[12], A[9], e4[4];
+                    int q;
+                    for (q = 0; q < 12; q++) gv[q] = g[(size_t)q * N + c];
+                    for (q = 0; q < 9; q++) A[q] = metrics[q][gg];
+                    orc_cell_vis_flux(gv, u[gg], v[gg], w[gg], amu[c], vis_c, A, ajac[gg], dir, e4);
+                    for (q = 0; q < 4; q++) ev[(size_t)q * NG + gg] = e4[q];
                 }
         for (m = 0; m < 4; m++) orc_ghost_fill_periodic(p, ev + (size_t)m * NG);
         for (m = 0; m < 4; m++) {

@@ -249,4 +249,67 @@ static inline orc_real orc_cell_sound_speed(orc_real T, double ama)
     return ORC_SQRT(T) / ama;
 }
 
+/* ---- viscous terms ---------------------------------------------------------------------- */
+
+/* Central first derivative of one stencil, not yet divided by h: CD6 (s[0..6] centred on s[3],
+ * src/kernels/OCFD_Schemes.cpp:99-105) and CD8 (s[0..8] centred on s[4], :130-138). */
+static inline orc_real orc_cell_cd6(const orc_real *s)
+{
+    return (45.0*(s[4] - s[2]) - 9.0*(s[5] - s[1]) + (s[6] - s[0])) / 60.0;
+}
+
+static inline orc_real orc_cell_cd8(const orc_real *s)
+{
+    return (672.0*(s[5] - s[3]) - 168.0*(s[6] - s[2]) + 32.0*(s[7] - s[1]) - 3.0*(s[8] - s[0])) / 840.0;
+}
+
+/* Sutherland viscosity of one cell (get_Amu_kernal, src/kernels/commen_kernel.cpp:123-141);
+ * c0 = (1 + 110.4/Ref_T)/Re and tsb = 110.4/Ref_T are run constants. */
+static inline orc_real orc_cell_amu(orc_real t, double c0, double tsb)
+{
+    return c0 * ORC_SQRT(t*t*t) / (tsb + t);
+}
+
+/* The four viscous fluxes of direction dir at one cell (vis_flux_ker,
+ * src/solver/OCFD_NS_Jacobian3d.cpp:477-620): g[4*d + var] the cell's gradients along
+ * computational direction d (var = u, v, w, T), A[0..8] its metric terms Akx, Aky, Akz, Aix,
+ * Aiy, Aiz, Asx, Asy, Asz, aj1 its Ajac, a1 its viscosity, vis_c = Cp/Pr.  The stress and heat
+ * flux are formed again for every direction, as the reference does. */
+static inline void orc_cell_vis_flux(const orc_real g[12], orc_real u, orc_real v, orc_real w, orc_real a1,
+                                     double vis_c, const orc_real A[9], orc_real aj1, int dir, orc_real ev[4])
+{
+    orc_real ux, vx, wx, uy, vy, wy, uz, vz, wz, div;
+    orc_real s11, s12, s13, s22, s23, s33, amuk, Tx, Ty, Tz, E1, E2, E3, akx, aky, akz;
+    ux = g[0] * A[0] + g[4] * A[3] + g[8] * A[6];
+    vx = g[1] * A[0] + g[5] * A[3] + g[9] * A[6];
+    wx = g[2] * A[0] + g[6] * A[3] + g[10] * A[6];
+    uy = g[0] * A[1] + g[4] * A[4] + g[8] * A[7];
+    vy = g[1] * A[1] + g[5] * A[4] + g[9] * A[7];
+    wy = g[2] * A[1] + g[6] * A[4] + g[10] * A[7];
+    uz = g[0] * A[2] + g[4] * A[5] + g[8] * A[8];
+    vz = g[1] * A[2] + g[5] * A[5] + g[9] * A[8];
+    wz = g[2] * A[2] + g[6] * A[5] + g[10] * A[8];
+    div = ux + vy + wz;
+    s11 = (2.0 * ux - 2.0 / 3.0 * div) * a1;
+    s22 = (2.0 * vy - 2.0 / 3.0 * div) * a1;
+    s33 = (2.0 * wz - 2.0 / 3.0 * div) * a1;
+    s12 = (uy + vx) * a1;
+    s13 = (uz + wx) * a1;
+    s23 = (vz + wy) * a1;
+    amuk = a1 * vis_c;
+    Tx = g[3] * A[0] + g[7] * A[3] + g[11] * A[6];
+    Ty = g[3] * A[1] + g[7] * A[4] + g[11] * A[7];
+    Tz = g[3] * A[2] + g[7] * A[5] + g[11] * A[8];
+    E1 = u * s11 + v * s12 + w * s13 + amuk * Tx;
+    E2 = u * s12 + v * s22 + w * s23 + amuk * Ty;
+    E3 = u * s13 + v * s23 + w * s33 + amuk * Tz;
+    akx = A[3 * dir] * aj1;
+    aky = A[3 * dir + 1] * aj1;
+    akz = A[3 * dir + 2] * aj1;
+    ev[0] = akx * s11 + aky * s12 + akz * s13;
+    ev[1] = akx * s12 + aky * s22 + akz * s23;
+    ev[2] = akx * s13 + aky * s23 + akz * s33;
+    ev[3] = akx * E1 + aky * E2 + akz * E3;
+}
+
 #endif

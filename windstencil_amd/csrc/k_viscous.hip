@@ -510,7 +510,9 @@ template <int ORDER, bool BND>
 __global__ void __launch_bounds__(FNT) k_vis_fused(VisParams p, ws_field u, ws_field v, ws_field w, ws_field T,
                                                    ws_soa du, int zc)
 {
-#pragma clang fp contract(off)
+    // contraction stays on here (the fast family's arithmetic need not follow the reference's
+    // order): every thread runs the same instructions for its flux point, so the flux at a
+    // point past a periodic face still equals the one at its image bit for bit
     constexpr int R = ORDER / 2, Q = 2 * R + 1;
     using FZ = Fused<R>;
     __shared__ double pl[4][FZ::PY][FZ::PX];      // u, v, w, T of the plane: flux points and border
@@ -606,7 +608,7 @@ __global__ void __launch_bounds__(FNT) k_vis_fused(VisParams p, ws_field u, ws_f
             const double us = cd_at<ORDER, false, BND>(c[0], p, 2, zf), vs = cd_at<ORDER, false, BND>(c[1], p, 2, zf);
             const double ws = cd_at<ORDER, false, BND>(c[2], p, 2, zf), Ts = cd_at<ORDER, false, BND>(c[3], p, 2, zf);
             const double tc = c[3][R];
-            const double amu = p.amu_c0 * sqrt(tc * tc * tc) / (p.tsb + tc);
+            const double amu = p.amu_c0 * sqrt_nr(tc * tc * tc) * recip_nr(p.tsb + tc);
             const double ux = uk * Akx + ui * Aix + us * Asx, vx = vk * Akx + vi * Aix + vs * Asx;
             const double wx = wk * Akx + wi * Aix + ws * Asx;
             const double uy = uk * Aky + ui * Aiy + us * Asy, vy = vk * Aky + vi * Aiy + vs * Asy;

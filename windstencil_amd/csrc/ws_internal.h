@@ -76,7 +76,7 @@ struct Ctx {
     // flux fields, allocated at the first use; planes per work-group of the fused form
     double *vis_scratch = nullptr;
     size_t vis_scratch_elems = 0;
-    int vis_zchunk = 32;
+    int vis_zchunk = 64;
     // second set of ghosted primitive fields (d, u, v, w, T, cc) that RK3 stages 1 and 2
     // write their primitives to, allocated at the first eager RK3 step that can use it
     double *prim_scratch = nullptr;
