@@ -377,7 +377,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 "
-                                   "(the multi-GPU decomposition is a design only, see DESIGN.md)"}))
+                                   "(the multi-GPU decomposition is a design only)"}))
         sys.exit(2)
     import torch
     import windstencil_amd as W

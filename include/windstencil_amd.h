@@ -132,7 +132,13 @@ typedef struct ws_config {
     double re, pr, ref_t;        /* Re, Pr, Ref_T of the viscous terms (Sutherland viscosity) */
     int32_t d0_bound[6];         /* 1: one-sided closure of the inviscid scheme at that domain face (x-, x+,
                                     y-, y+, z-, z+; the reference's D0_bound, src/boundary/OCFD_bound_Scheme.cpp:
-                                    432-893); needs periodic[d] = 0 and at least 8 cells */
+                                    432-893); needs periodic[d] = 0 and at least 8 cells.  2 (high faces only):
+                                    the form of the fused path's rim kernel (split_Jac3d_Stager_Warming_ker_
+                                    boundary, src/solver/OCFD_NS_Jacobian3d.cpp:1063-1085), as 1 except the last
+                                    P face 1.5 f(n-1) - 0.5 f(n-2) and the last two M faces 0.5 f(n-1) +
+                                    0.5 f(n-2).  The rim kernel's x closures are {1, 2}, its y closures {1, 1}
+                                    with non_ref of the y M pass (it drops the last cell's M difference); the
+                                    viscous terms treat 2 as 1 */
     int32_t non_ref[6];          /* 1: non-reflecting flag of the P (even index) / M (odd) pass of a
                                     direction (the reference's Non_ref: the P pass skips the first cell, the M
                                     pass the last, where that side has the closure) */

@@ -300,7 +300,9 @@ static double w5_m_right_plus(const double *s)
 /* P face f (between cells f-1 and f) of a line with n cells and one-sided closures at the
  * low (lo) / high (hi) end, OCFD_bound_scheme_kernel_p (OCFD_bound_Scheme.cpp:722-807): the
  * kernel's thread c = f holds cell f-1 and the window s(f-4 .. f+3); the 5-point closures
- * use cells f-3 .. f+1.  s points at cell 0.  Returns 1 if a closure applies. */
+ * use cells f-3 .. f+1.  s points at cell 0.  Returns 1 if a closure applies.  hi == 2: the
+ * high end of the fused path's rim kernel (split_Jac3d_Stager_Warming_ker_boundary,
+ * src/solver/OCFD_NS_Jacobian3d.cpp:1063-1072), whose last face is 1.5 s(n-1) - 0.5 s(n-2). */
 static int closure_p(const double *s, long n, long f, int lo, int hi, double *out)
 {
     if (lo && f <= 3) {
@@ -311,7 +313,8 @@ static int closure_p(const double *s, long n, long f, int lo, int hi, double *ou
         return 1;
     }
     if (hi && f >= n - 2) {
-        if (f == n) *out = s[n - 1] + 0.5 * bs_minmod2(s[n - 1] - s[n - 2], s[n - 1] - s[n - 2]);
+        if (f == n && hi == 2) *out = 1.5 * s[n - 1] - 0.5 * s[n - 2];
+        else if (f == n) *out = s[n - 1] + 0.5 * bs_minmod2(s[n - 1] - s[n - 2], s[n - 1] - s[n - 2]);
         else if (f == n - 1) *out = w5_p_right_plus(s + f - 3);
         else *out = w5_p(s + f - 3);
         return 1;
@@ -320,7 +323,9 @@ static int closure_p(const double *s, long n, long f, int lo, int hi, double *ou
 }
 
 /* M face f, OCFD_bound_scheme_kernel_m (OCFD_bound_Scheme.cpp:808-893): thread c = f holds
- * cell f and the window s(f-4 .. f+3); the 5-point closures use cells f-2 .. f+2. */
+ * cell f and the window s(f-4 .. f+3); the 5-point closures use cells f-2 .. f+2.  hi == 2:
+ * the rim kernel's high end (OCFD_NS_Jacobian3d.cpp:1075-1085), whose last two faces both
+ * take 0.5 s(n-1) + 0.5 s(n-2). */
 static int closure_m(const double *s, long n, long f, int lo, int hi, double *out)
 {
     if (lo && f <= 3) {
@@ -330,7 +335,8 @@ static int closure_m(const double *s, long n, long f, int lo, int hi, double *ou
         return 1;
     }
     if (hi && f >= n - 3) {
-        if (f == n) *out = s[n - 1];
+        if (hi == 2 && f >= n - 1) *out = 0.5 * s[n - 1] + 0.5 * s[n - 2];
+        else if (f == n) *out = s[n - 1];
         else if (f == n - 1) *out = s[n - 1] - 0.5 * bs_minmod2(s[n - 1] - s[n - 2], s[n - 1] - s[n - 2]);
         else if (f == n - 2) *out = w5_m_right_plus(s + f - 2);
         else *out = w5_m(s + f - 2);

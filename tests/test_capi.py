@@ -70,6 +70,12 @@ def test_host_side_validation_without_gpu(ws):
     bad = ws.default_config(64, 32, 16, d0_bound=(0, 0, 1, 0, 0, 0))   # closure on a periodic direction
     assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -1
     assert b"non-periodic" in L.ws_last_error()
+    # the rim kernel's form (2) exists on high faces only
+    bad = ws.default_config(64, 32, 16, periodic=(0, 1, 1), d0_bound=(2, 1, 0, 0, 0, 0))
+    assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -1
+    assert b"high face" in L.ws_last_error()
+    bad = ws.default_config(64, 32, 16, periodic=(0, 1, 1), d0_bound=(1, 3, 0, 0, 0, 0))
+    assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -1
     # viscous terms: every direction periodic or closed on both faces
     bad = ws.default_config(64, 32, 16, periodic=(1, 0, 1), viscous=6)
     assert L.ws_ctx_create(ctypes.byref(bad), ctypes.byref(h)) == -2

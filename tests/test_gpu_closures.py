@@ -1,5 +1,5 @@
-"""GPU parity of the one-sided boundary closures (the reference's D0_bound path) and the
-non-reflecting flags: non-periodic directions keep the caller's ghost layers (filled here
+"""GPU parity of the one-sided boundary closures (the reference's D0_bound path and the high-
+face form of the fused path's rim kernel) and the non-reflecting flags: non-periodic directions keep the caller's ghost layers (filled here
 with arbitrary valid values, given identically to the oracle).  Anchor kernels bitwise,
 tiled FP64 within 1e-12 for every tile shape and direction order, mixed within 1e-6."""
 import numpy as np
@@ -55,6 +55,10 @@ CASES = [
     ("x high only + y both", (0, 0, 1), (0, 1, 1, 1, 0, 0), (0,) * 6),
     ("x non-reflecting", (0, 1, 1), (1, 1, 0, 0, 0, 0), (1, 1, 0, 0, 0, 0)),
     ("all closed, non-reflecting", (0, 0, 0), (1,) * 6, (1, 0, 0, 1, 1, 1)),
+    # the rim kernel's high-face form, and the fused path's closures as a whole: x {1, 2},
+    # y {1, 1} with the M pass of y dropped in the last cell, z from the ghosts
+    ("x high rim form", (0, 1, 1), (1, 2, 0, 0, 0, 0), (0,) * 6),
+    ("fused-path rim closures", (0, 0, 1), (1, 2, 1, 1, 0, 0), (0, 0, 0, 1, 0, 0)),
 ]
 
 

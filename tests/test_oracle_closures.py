@@ -1,6 +1,7 @@
 """Pins of the oracle's one-sided boundary closures (the reference's D0_bound path,
-src/boundary/OCFD_bound_Scheme.cpp:540-893) and non-reflecting flags: which faces they
-cover, exactness on linear data, the use-before-set weights of OCFD_weno5_kernel_P,
+src/boundary/OCFD_bound_Scheme.cpp:540-893, and the high-face form of the fused path's rim
+kernel, src/solver/OCFD_NS_Jacobian3d.cpp:1063-1085) and non-reflecting flags: which faces
+they cover, exactness on linear data, the use-before-set weights of OCFD_weno5_kernel_P,
 telescoping of the RHS to the boundary faces, and which ghost values the RHS reads."""
 import numpy as np
 import pytest
@@ -139,3 +140,56 @@ def test_non_reflecting_flags_drop_one_pass_at_the_ends():
     d1 = O.du_comput(O.params(nx, ny, nz, non_ref=(1, 1, 0, 0, 0, 0), **base_kw), *[a.copy() for a in g])[0]
     diff = np.nonzero(np.any(d0 != d1, axis=(0, 1, 2)))[0]
     assert set(diff.tolist()) == {0, nx - 1}
+
+
+def test_rim_form_differs_only_at_the_last_faces():
+    """d0_bound 2 on a high face (the rim kernel's closures of x): the same faces are covered
+    and every face value equals the D0_bound one except P at face n and M at faces n-1 and
+    n.  Those are fixed by the mathematics of their stencils: on linear data P(n) is the
+    exact value at the face (linear extrapolation, as the D0_bound form's slope-limited
+    extrapolation is for monotone data: the two agree to rounding), M(n-1) the exact value
+    at face n-1, and M(n) repeats M(n-1) bit for bit -- so the last cell's M difference is
+    exactly zero, where D0_bound takes the first-order value of cell n-1."""
+    n = 16
+    rng = np.random.default_rng(11)
+    for trial in range(30):
+        line = rng.standard_normal(n + 8) if trial else 0.7 + 0.3 * np.arange(-4, n + 4, dtype=float)
+        for side in ("p", "m"):
+            for f in range(n + 1):
+                a, b = O.closure(side, line, f, hi=1), O.closure(side, line, f, hi=2)
+                assert (a is None) == (b is None), (side, f)
+                if a is not None and not ((side == "p" and f == n) or (side == "m" and f >= n - 1)):
+                    assert a == b, (side, f)
+        a1, a2 = line[4 + n - 1], line[4 + n - 2]
+        mono = (a1 - a2) != 0
+        if mono:
+            assert abs(O.closure("p", line, n, hi=2) - O.closure("p", line, n, hi=1)) <= 1e-14 * (abs(a1) + abs(a2))
+        assert O.closure("m", line, n, hi=2) == O.closure("m", line, n - 1, hi=2)
+        if trial == 0:                        # linear data: face f sits at x = f - 1/2
+            assert abs(O.closure("p", line, n, hi=2) - (0.7 + 0.3 * (n - 0.5))) <= 1e-13
+            assert abs(O.closure("m", line, n - 1, hi=2) - (0.7 + 0.3 * (n - 1.5))) <= 1e-13
+            assert O.closure("m", line, n, hi=1) == a1
+
+
+def test_rim_form_changes_only_the_last_two_cells():
+    """With the rim kernel's x closures (d0_bound x = {1, 2}) the RHS differs from the
+    D0_bound one only in the last two x cells (always in the last), and the M pass of the
+    last cell contributes nothing."""
+    nx, ny, nz = 20, 6, 6
+    prims = random_state(nx, ny, nz, 4, "shock")
+    g = [ghosted(a) for a in prims]
+    for a in g:
+        O.ghost_fill_periodic(O.params(nx, ny, nz), a)
+    d1 = O.du_comput(O.params(nx, ny, nz, periodic=(0, 1, 1), d0_bound=(1, 1, 0, 0, 0, 0)), *[a.copy() for a in g])[0]
+    d2 = O.du_comput(O.params(nx, ny, nz, periodic=(0, 1, 1), d0_bound=(1, 2, 0, 0, 0, 0)), *[a.copy() for a in g])[0]
+    diff = np.nonzero(np.any(d1 != d2, axis=(0, 1, 2)))[0]
+    # cell n-2 changes only through M(n-1), whose two forms agree in exact arithmetic and
+    # may round alike
+    assert nx - 1 in diff.tolist() and set(diff.tolist()) <= {nx - 2, nx - 1}
+    # the last cell: rim = D0_bound with its M pass dropped and P(n) replaced (the M faces
+    # n-1 and n are equal); compare against the D0_bound run with the M flag of x set,
+    # which drops the same M pass: they differ only through P(n), by rounding
+    d3 = O.du_comput(O.params(nx, ny, nz, periodic=(0, 1, 1), d0_bound=(1, 1, 0, 0, 0, 0), non_ref=(0, 1, 0, 0, 0, 0)),
+                     *[a.copy() for a in g])[0]
+    last = (slice(None), slice(None), slice(None), nx - 1)
+    assert np.allclose(d2[last], d3[last], rtol=1e-12, atol=1e-12 * np.abs(d3).max())

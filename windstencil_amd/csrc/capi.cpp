@@ -47,8 +47,9 @@ KParams make_kparams(const ws_config &c)
         k.Ak[3 * dir] = a1 / sn; k.Ak[3 * dir + 1] = a2 / sn; k.Ak[3 * dir + 2] = a3 / sn;
     }
     k.ajac = c.metric[9];
-    for (int i = 0; i < 6; ++i) { k.bnd[i] = c.d0_bound[i] ? 1 : 0; k.nonref[i] = c.non_ref[i] ? 1 : 0; }
-    for (int d = 0; d < 3; ++d) k.bnd_any[d] = k.bnd[2 * d] | k.bnd[2 * d + 1];
+    // bnd: the closure form of each face (0 none, 1 D0_bound, 2 the rim kernel's high face)
+    for (int i = 0; i < 6; ++i) { k.bnd[i] = c.d0_bound[i]; k.nonref[i] = c.non_ref[i] ? 1 : 0; }
+    for (int d = 0; d < 3; ++d) k.bnd_any[d] = (k.bnd[2 * d] | k.bnd[2 * d + 1]) != 0;
     return k;
 }
 
@@ -125,8 +126,9 @@ int validate_config(const ws_config *cfg)
             return set_error(WS_ERR_INVALID, "periodic[d] must be 0 (caller's ghosts) or 1");
         for (int side = 0; side < 2; ++side) {
             const int b = cfg->d0_bound[2 * i + side], nr = cfg->non_ref[2 * i + side];
-            if ((b != 0 && b != 1) || (nr != 0 && nr != 1))
-                return set_error(WS_ERR_INVALID, "d0_bound and non_ref entries must be 0 or 1");
+            if ((b != 0 && b != 1 && !(side == 1 && b == 2)) || (nr != 0 && nr != 1))
+                return set_error(WS_ERR_INVALID, "d0_bound entries must be 0 or 1 (2 also on a high face: the rim "
+                                                 "kernel's form), non_ref entries 0 or 1");
             if (b && cfg->periodic[i])
                 return set_error(WS_ERR_INVALID, "a one-sided closure (d0_bound) needs a non-periodic direction");
         }

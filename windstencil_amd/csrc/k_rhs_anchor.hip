@@ -107,7 +107,8 @@ __global__ void __launch_bounds__(256) k_anchor_weno(KParams k, ws_metric_fields
     const int n_dom = DIR == 0 ? k.nx : DIR == 1 ? k.ny : k.nz;
     const int g0 = box.s[DIR] - LAP;                 // global index of the box's first cell
     const int fl = g0 + cpos, fr = fl + 1;
-    const bool lo = k.bnd[2 * DIR] != 0, hi = k.bnd[2 * DIR + 1] != 0;
+    const bool lo = k.bnd[2 * DIR] != 0;
+    const int hi = k.bnd[2 * DIR + 1];   // the high face's closure form (0: none)
     const bool skipP = lo && k.nonref[2 * DIR] && fl == 0;
     const bool skipM = hi && k.nonref[2 * DIR + 1] && fl == n_dom - 1;
 #pragma unroll 1

@@ -240,7 +240,8 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     // one-sided closures (BND): global index of the tile's first cell along the line
     const int n_dom = DIR == 0 ? k.nx : DIR == 1 ? k.ny : k.nz;
     const int gl0 = (DIR == 0 ? g.x0 : DIR == 1 ? g.y0 : g.z0) - LAP;
-    const bool lo = BND && k.bnd[2 * DIR], hi = BND && k.bnd[2 * DIR + 1];
+    const bool lo = BND && k.bnd[2 * DIR];
+    const int hi = BND ? k.bnd[2 * DIR + 1] : 0;     // the high face's closure form (0: none)
     const bool nr_lo = lo && k.nonref[2 * DIR], nr_hi = hi && k.nonref[2 * DIR + 1];
     double nfix[BND ? 5 : 1];       // P(n) - P(n-1) of the line's last cell (non-reflecting M)
     // MIX: bit m*K + j set when face j of variable m had FP32 weights out of FP32's range

@@ -508,9 +508,12 @@ __device__ __forceinline__ double w5_S2p(double s0, double s1, double s2)
 // P face f (between cells f-1 and f) of a line of n cells; cell(c) gives the split value
 // of cell c.  Returns true and the face value when a closure covers the face
 // (OCFD_bound_scheme_kernel_p, OCFD_bound_Scheme.cpp:722-807; 5-point windows on cells
-// f-3 .. f+1).
+// f-3 .. f+1).  hi: the high face's form, 0 none, 1 D0_bound, 2 the fused path's rim kernel
+// (split_Jac3d_Stager_Warming_ker_boundary, src/solver/OCFD_NS_Jacobian3d.cpp:1063-1085),
+// which extrapolates the last face linearly, 1.5 f(n-1) - 0.5 f(n-2), where D0_bound
+// limits the slope with minmod (the same value in exact arithmetic, other roundings).
 template <typename Cell>
-__device__ __forceinline__ bool closure_p(Cell cell, int n, int f, bool lo, bool hi, double &out)
+__device__ __forceinline__ bool closure_p(Cell cell, int n, int f, bool lo, int hi, double &out)
 {
 #pragma clang fp contract(off)
     if (lo && f <= 3) {
@@ -531,7 +534,7 @@ __device__ __forceinline__ bool closure_p(Cell cell, int n, int f, bool lo, bool
     if (hi && f >= n - 2) {
         if (f == n) {
             const double a = cell(n - 1), b = cell(n - 2);
-            out = a + 0.5 * bs_minmod2(a - b, a - b);
+            out = hi == 2 ? 1.5 * a - 0.5 * b : a + 0.5 * bs_minmod2(a - b, a - b);
         } else if (f == n - 1) {        // right_plus: candidates 1 and 2
             const double s0 = cell(f - 3), s1 = cell(f - 2), s2 = cell(f - 1), s3 = cell(f);
             out = w5_two(w5_S1p(s1, s2, s3), (-s1 + 5.0 * s2 + 2.0 * s3), w5_S2p(s0, s1, s2),
@@ -545,9 +548,11 @@ __device__ __forceinline__ bool closure_p(Cell cell, int n, int f, bool lo, bool
 }
 
 // M face f (OCFD_bound_scheme_kernel_m, OCFD_bound_Scheme.cpp:808-893; windows on cells
-// f-2 .. f+2, mirrored candidates)
+// f-2 .. f+2, mirrored candidates).  With the rim kernel's form (hi == 2,
+// OCFD_NS_Jacobian3d.cpp:1075-1085) the last two faces both take 0.5 f(n-1) + 0.5 f(n-2),
+// so the last cell's M difference vanishes.
 template <typename Cell>
-__device__ __forceinline__ bool closure_m(Cell cell, int n, int f, bool lo, bool hi, double &out)
+__device__ __forceinline__ bool closure_m(Cell cell, int n, int f, bool lo, int hi, double &out)
 {
 #pragma clang fp contract(off)
     if (lo && f <= 3) {
@@ -572,10 +577,10 @@ __device__ __forceinline__ bool closure_m(Cell cell, int n, int f, bool lo, bool
     }
     if (hi && f >= n - 3) {
         if (f == n) {
-            out = cell(n - 1);
+            out = hi == 2 ? 0.5 * cell(n - 1) + 0.5 * cell(n - 2) : cell(n - 1);
         } else if (f == n - 1) {
             const double a = cell(n - 1), b = cell(n - 2);
-            out = a - 0.5 * bs_minmod2(a - b, a - b);
+            out = hi == 2 ? 0.5 * a + 0.5 * b : a - 0.5 * bs_minmod2(a - b, a - b);
         } else if (f == n - 2) {        // right_plus: candidates 0 and 1
             const double s0 = cell(f - 2), s1 = cell(f - 1), s2 = cell(f), s3 = cell(f + 1);
             double S0 = 0.0, S1 = 0.0, tmp;

@@ -30,9 +30,10 @@ struct KParams {
     double Anorm[3];
     double Ak[9];
     double ajac;
-    // one-sided closures of the inviscid scheme at the domain faces (the reference's
-    // D0_bound, x-, x+, y-, y+, z-, z+) and the non-reflecting flags of the P (even index)
-    // and M (odd) passes of each direction (Non_ref); bnd_any[d]: either side of d closed
+    // one-sided closures of the inviscid scheme at the domain faces (x-, x+, y-, y+, z-, z+:
+    // 0 none, 1 the reference's D0_bound, 2 on a high face the rim kernel's form) and the
+    // non-reflecting flags of the P (even index) and M (odd) passes of each direction
+    // (Non_ref); bnd_any[d]: either side of d closed
     int bnd[6];
     int nonref[6];
     int bnd_any[3];
