@@ -2,8 +2,9 @@
 
 The FP32 reconstruction forms its weights as products of four (eps + S_k) factors with
 the indicators scaled by 1/720 (eps = 2.592e-6/720); the kernel recomputes a face in FP64
-when the weight sum D or its reciprocal is not a normal FP32 number (D outside
-[2^-126, 2^125]).  This audit evaluates D in FP64 from the oracle's indicators on every
+when |N| + D exceeds 2^125 (N the weighted numerator), which covers a weight sum D whose
+reciprocal is not a normal FP32 number; D itself never falls below eps^4, far above
+2^-126 (pinned below).  This audit evaluates D in FP64 from the oracle's indicators on every
 x-window of the positive split fluxes: none of the Sod-plus-noise windows comes near the
 limit (the fallback costs the benchmark nothing), while the blasts with pressure ratios of
 1e3 and more have windows beyond it (the GPU blast test exercises the fallback)."""

@@ -322,7 +322,7 @@ __device__ __forceinline__ void face64(const double *q, double &num, double &den
 // d_4 eps^4 (about 5e-36 with eps = 3.6e-9), so it never falls below FP32's normal range,
 // but it overflows once the window's first differences reach a few hundred (a strong
 // discontinuity of the split flux); N may overflow a little earlier.  Bit j of `bad` is set
-// when face j's D or 1/D is not a normal number or its N is not finite: the caller then
+// unless |N| + D <= 2^125 (so that D, 1/D and N are normal numbers): the caller then
 // recomputes that face in FP64 (weno_k), so FP32 weights are used only where they are
 // representable.  Below that limit the FP32 correction's error stays relative to the
 // window's variation (no catastrophic cancellation: the indicators are sums of squares
@@ -400,9 +400,9 @@ __device__ __forceinline__ void weno_k_mixed(Load ld, double (&base)[K], float (
         D = fmaf(D, e4, fD4 * p0123);
         corr[j] = N * __builtin_amdgcn_rcpf(D);
         base[j] = REV ? s[j + 4] : s[j + 3];
-        // D in [2^-126, 2^125], so that D and its reciprocal are normal (false for NaN), and
-        // |N| finite
-        const bool ok = D >= 1.17549435e-38f && D <= 4.25352958e+37f && fabsf(N) <= 3.40282347e+38f;
+        // D >= eps^4 > 2^-126 always, so D and 1/D are normal and N finite when |N| + D <=
+        // 2^125 (false for NaN; a finite |N| above 2^125 is sent to FP64 as well)
+        const bool ok = fabsf(N) + D <= 4.25352958e+37f;
         bad |= ok ? 0u : (1u << j);
     }
 }
