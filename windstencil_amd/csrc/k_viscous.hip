@@ -506,7 +506,10 @@ __device__ __forceinline__ int wrap_coord(int c, int n, bool periodic)
     return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
 
-template <int ORDER, bool BND>
+// DIAG: the constant metrics have zero off-diagonal terms (a Cartesian box), so each metric
+// product keeps only its diagonal term; the values are the general form's (a zero product
+// adds nothing to a sum or an FMA), with about 10 % fewer instructions.
+template <int ORDER, bool BND, bool DIAG>
 __global__ void __launch_bounds__(FNT) k_vis_fused(VisParams p, ws_field u, ws_field v, ws_field w, ws_field T,
                                                    ws_soa du, int zc)
 {
@@ -609,31 +612,40 @@ __global__ void __launch_bounds__(FNT) k_vis_fused(VisParams p, ws_field u, ws_f
             const double ws = cd_at<ORDER, false, BND>(c[2], p, 2, zf), Ts = cd_at<ORDER, false, BND>(c[3], p, 2, zf);
             const double tc = c[3][R];
             const double amu = p.amu_c0 * sqrt_nr(tc * tc * tc) * recip_nr(p.tsb + tc);
-            const double ux = uk * Akx + ui * Aix + us * Asx, vx = vk * Akx + vi * Aix + vs * Asx;
-            const double wx = wk * Akx + wi * Aix + ws * Asx;
-            const double uy = uk * Aky + ui * Aiy + us * Asy, vy = vk * Aky + vi * Aiy + vs * Asy;
-            const double wy = wk * Aky + wi * Aiy + ws * Asy;
-            const double uz = uk * Akz + ui * Aiz + us * Asz, vz = vk * Akz + vi * Aiz + vs * Asz;
-            const double wz = wk * Akz + wi * Aiz + ws * Asz;
+            // physical derivatives from the computational ones (k, i, s)
+            auto gx = [&](double gk, double gi, double gs) { return DIAG ? gk * Akx : gk * Akx + gi * Aix + gs * Asx; };
+            auto gy = [&](double gk, double gi, double gs) { return DIAG ? gi * Aiy : gk * Aky + gi * Aiy + gs * Asy; };
+            auto gz = [&](double gk, double gi, double gs) { return DIAG ? gs * Asz : gk * Akz + gi * Aiz + gs * Asz; };
+            const double ux = gx(uk, ui, us), vx = gx(vk, vi, vs), wx = gx(wk, wi, ws);
+            const double uy = gy(uk, ui, us), vy = gy(vk, vi, vs), wy = gy(wk, wi, ws);
+            const double uz = gz(uk, ui, us), vz = gz(vk, vi, vs), wz = gz(wk, wi, ws);
             const double div = ux + vy + wz;
             const double s11 = (2.0 * ux - 2.0 / 3.0 * div) * amu;
             const double s22 = (2.0 * vy - 2.0 / 3.0 * div) * amu;
             const double s33 = (2.0 * wz - 2.0 / 3.0 * div) * amu;
             const double s12 = (uy + vx) * amu, s13 = (uz + wx) * amu, s23 = (vz + wy) * amu;
             const double amuk = amu * p.vis_c;
-            const double Tx = Tk * Akx + Ti * Aix + Ts * Asx;
-            const double Ty = Tk * Aky + Ti * Aiy + Ts * Asy;
-            const double Tz = Tk * Akz + Ti * Aiz + Ts * Asz;
+            const double Tx = gx(Tk, Ti, Ts), Ty = gy(Tk, Ti, Ts), Tz = gz(Tk, Ti, Ts);
             const double uc = c[0][R], vc = c[1][R], wc = c[2][R];
             const double E1 = uc * s11 + vc * s12 + wc * s13 + amuk * Tx;
             const double E2 = uc * s12 + vc * s22 + wc * s23 + amuk * Ty;
             const double E3 = uc * s13 + vc * s23 + wc * s33 + amuk * Tz;
             auto flux = [&](int d, double out[4]) {
-                const double akx = p.metric[3 * d] * aj, aky = p.metric[3 * d + 1] * aj, akz = p.metric[3 * d + 2] * aj;
-                out[0] = akx * s11 + aky * s12 + akz * s13;
-                out[1] = akx * s12 + aky * s22 + akz * s23;
-                out[2] = akx * s13 + aky * s23 + akz * s33;
-                out[3] = akx * E1 + aky * E2 + akz * E3;
+                if constexpr (DIAG) {
+                    // row d of the metrics keeps its diagonal term A[4d] only
+                    const double a = p.metric[4 * d] * aj;
+                    out[0] = a * (d == 0 ? s11 : d == 1 ? s12 : s13);
+                    out[1] = a * (d == 0 ? s12 : d == 1 ? s22 : s23);
+                    out[2] = a * (d == 0 ? s13 : d == 1 ? s23 : s33);
+                    out[3] = a * (d == 0 ? E1 : d == 1 ? E2 : E3);
+                } else {
+                    const double akx = p.metric[3 * d] * aj, aky = p.metric[3 * d + 1] * aj;
+                    const double akz = p.metric[3 * d + 2] * aj;
+                    out[0] = akx * s11 + aky * s12 + akz * s13;
+                    out[1] = akx * s12 + aky * s22 + akz * s23;
+                    out[2] = akx * s13 + aky * s23 + akz * s33;
+                    out[3] = akx * E1 + aky * E2 + akz * E3;
+                }
             };
             double F[4];
             if (xrow) {
@@ -769,12 +781,19 @@ int launch_viscous(Ctx &c, ws_field u, ws_field v, ws_field w, ws_field T, const
         // the fast family with constant metrics: one fused kernel, no scratch
         const int zc = c.vis_zchunk;
         const unsigned g = fused_blocks(cfg.nx, cfg.ny, cfg.nz, cfg.viscous, zc);
-        if (cfg.viscous == 8) {
-            if (bnd) hipLaunchKernelGGL((k_vis_fused<8, true>), dim3(g), dim3(FNT), 0, s, vp, u, v, w, T, du, zc);
-            else hipLaunchKernelGGL((k_vis_fused<8, false>), dim3(g), dim3(FNT), 0, s, vp, u, v, w, T, du, zc);
-        } else {
-            if (bnd) hipLaunchKernelGGL((k_vis_fused<6, true>), dim3(g), dim3(FNT), 0, s, vp, u, v, w, T, du, zc);
-            else hipLaunchKernelGGL((k_vis_fused<6, false>), dim3(g), dim3(FNT), 0, s, vp, u, v, w, T, du, zc);
+        const double *A = cfg.metric;
+        const bool diag = A[1] == 0.0 && A[2] == 0.0 && A[3] == 0.0 && A[5] == 0.0 && A[6] == 0.0 && A[7] == 0.0;
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(g), dim3(FNT), 0, s, vp, u, v, w, T, du, zc); };
+        const int sel = (cfg.viscous == 8 ? 4 : 0) + (bnd ? 2 : 0) + (diag ? 1 : 0);
+        switch (sel) {
+        case 0: go(k_vis_fused<6, false, false>); break;
+        case 1: go(k_vis_fused<6, false, true>); break;
+        case 2: go(k_vis_fused<6, true, false>); break;
+        case 3: go(k_vis_fused<6, true, true>); break;
+        case 4: go(k_vis_fused<8, false, false>); break;
+        case 5: go(k_vis_fused<8, false, true>); break;
+        case 6: go(k_vis_fused<8, true, false>); break;
+        default: go(k_vis_fused<8, true, true>); break;
         }
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? WS_OK : hip_error(e, "fused viscous launch");
