@@ -60,6 +60,49 @@ def test_viscous_terms(orc, ws, order, kind, kernel):
         assert err <= 1e-13
 
 
+CONST_METRICS = {
+    # a sheared constant metric set (every term of the metric products in play) and a
+    # non-identity diagonal one (the fused kernel's diagonal instantiation)
+    "sheared": (1.0, 0.2, -0.1, 0.15, 0.9, 0.05, -0.05, 0.12, 1.1, 0.8),
+    "diagonal": (2.0, 0.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 1.5, 0.7),
+}
+
+
+@pytest.mark.parametrize("kernel", [W.WS_KERNEL_ANCHOR, W.WS_KERNEL_TILED])
+@pytest.mark.parametrize("order", [6, 8])
+@pytest.mark.parametrize("mname", list(CONST_METRICS))
+def test_viscous_terms_constant_metrics(orc, ws, mname, order, kernel):
+    """Constant metrics given in the configuration (no metric fields): the oracle gets the
+    same values as uniform fields.  The tiled family runs the fused kernel's general form
+    for the sheared set and its diagonal form for the diagonal one."""
+    import torch
+    met = CONST_METRICS[mname]
+    nx, ny, nz = 29, 22, 19
+    prims = random_state(nx, ny, nz, 17, "blobs")
+    p = orc.params(nx, ny, nz)
+    g = [ghosted(a) for a in prims]
+    for a in g:
+        orc.ghost_fill_periodic(p, a)
+    fields = [np.full_like(g[0], m) for m in met]
+    ref = orc.viscous_rhs(p, g[1], g[2], g[3], g[4], RE, PR, TREF, order, metrics=fields[:9], ajac=fields[9])
+    ctx = make_ctx(nx, ny, nz, kernel=kernel, metric=met, **_vis_cfg(order))
+    st = upload_state(prims)
+    L = W.lib()
+    s = W.stream_handle()
+    fl = (W.WsField * 5)(st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T)
+    W.check(L.ws_ghost_fill_periodic(ctx.handle, fl, 5, s))
+    st.t["du"].zero_()
+    W.check(L.ws_viscous_rhs(ctx.handle, st.ws.u, st.ws.v, st.ws.w, st.ws.T, None, st.ws.du, s))
+    torch.cuda.synchronize()
+    du = st.interior("du").cpu().numpy()
+    err = rel_l2(du, ref)
+    log(f"viscous {mname} metrics order {order} kernel {kernel}: rel L2 {err:.3e}")
+    if kernel == W.WS_KERNEL_ANCHOR:
+        assert np.array_equal(du, ref)
+    else:
+        assert err <= 1e-13
+
+
 @pytest.mark.parametrize("shape", [(4, 4, 4), (5, 9, 4), (37, 6, 41)])
 def test_viscous_terms_small_and_ragged_extents(orc, ws, shape):
     """Extents at the halo minimum and ragged ones (partial column tiles, z chunks
