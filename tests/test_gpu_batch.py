@@ -243,3 +243,48 @@ def test_batch_autotune_keeps_results(ws):
     _sync()
     for a, st in zip(before, b.states):
         assert rel_l2(_du(st), a) <= 1e-13
+
+
+def test_batch_helpers_in_one_launch_equal_per_block_launches(ws):
+    """The helpers around the batched RHS (ghost fill, sound speed, the stage kernel) run as
+    one launch over all blocks once the batch's work-group table exists, which the first
+    call outside a stream capture builds; a batch whose first call is captured runs them
+    block by block.  The RHS and an RK3 step (boundary planes on, fixed dt) come out the
+    same bit for bit either way."""
+    import torch
+    L = W.lib()
+    extents = _extents(29, 6, 5, 40)
+    results = {}
+    for captured in (False, True):
+        outs = []
+        for call in ("du", "step"):
+            b, _ = _batch(extents, 900, bc_planes=1, dt=2e-4)
+            W.check(L.ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
+            for e, st in zip(extents, b.states):
+                _to_cons(make_ctx(*e), st)
+            _sync()
+
+            def run():
+                s = W.stream_handle()
+                if call == "du":
+                    W.check(L.ws_batch_du_comput(b.handle, s), "batch du_comput")
+                else:
+                    W.check(L.ws_batch_rk3_step(b.handle, None, s), "batch rk3 step")
+
+            if captured:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    run()
+                g.replay()
+            else:
+                run()
+            b.sync_states()
+            _sync()
+            outs.append([st.interior("du" if call == "du" else "f").cpu().numpy() for st in b.states])
+            b.close()
+        results[captured] = outs
+        log(f"batch helpers captured={captured} done")
+    for a, c in zip(results[False], results[True]):
+        for i, (x, y) in enumerate(zip(a, c)):
+            assert np.isfinite(x).all(), i
+            assert np.array_equal(x, y), i
