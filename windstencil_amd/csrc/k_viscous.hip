@@ -1,6 +1,8 @@
 // k_viscous.hip -- viscous terms of the RHS (the reference's du_vis_Jacobian3d path).
 //
-// Two kernels around a ghost fill, each marching column tiles along z:
+// The tiled kernel family with constant metrics runs one fused kernel, k_vis_fused (below),
+// whose fluxes never leave the chip.  The anchor family and per-cell metric fields run two
+// kernels around a ghost fill, each marching column tiles along z:
 //   k_vis_flux: per interior cell, Sutherland viscosity, the twelve central-difference
 //               gradients (u, v, w, T along x, y, z), the stress tensor and heat flux, and
 //               the four viscous fluxes of each of the three directions (twelve ghosted
