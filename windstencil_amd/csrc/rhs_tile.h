@@ -109,14 +109,20 @@ struct TileLoads {
     double v[Tile<DIR, L, W, K>::ROUNDS][5];
 };
 
-// point r of this thread: line l, point p (x fastest in memory for every direction)
+// point pt of the tile: line l, point p (x fastest in memory for every direction)
+template <int DIR, int L, int W, int K>
+__device__ __forceinline__ void tile_point_at(int pt, int &l, int &p)
+{
+    using TL = Tile<DIR, L, W, K>;
+    if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }
+    else          { l = pt % W;      p = pt / W; }
+}
+
+// point r of this thread
 template <int DIR, int L, int W, int K>
 __device__ __forceinline__ void tile_point(int r, int &l, int &p)
 {
-    using TL = Tile<DIR, L, W, K>;
-    const int pt = threadIdx.x + r * TL::NT;
-    if (DIR == 0) { p = pt % TL::NL; l = pt / TL::NL; }
-    else          { l = pt % W;      p = pt / W; }
+    tile_point_at<DIR, L, W, K>((int)threadIdx.x + r * Tile<DIR, L, W, K>::NT, l, p);
 }
 
 template <int DIR, int L, int W, int K>
@@ -136,24 +142,51 @@ __device__ __forceinline__ void tile_point_xyz(const TileGeo &g, int l, int p, i
     else               { x = g.x0 + l; y = g.y0; z = g.z0 - LAP + p; }
 }
 
-// Phase 1a: issue the loads of this thread's points (nothing waits on them here).
+// Cell coordinates of phase-3 cell c (0 .. W*L-1 in the tile's cell order) for loads that
+// every lane issues: clamped to the tile's last cell and, across and along the line, to
+// its cells inside the box (the values of a clamped lane are never used).
+template <int DIR, int L, int W, int K>
+__device__ __forceinline__ void tile_cell_clamped(const TileGeo &g, int c, int &cx, int &cy, int &cz)
+{
+    c = min(c, W * L - 1);
+    int l, i;
+    if (DIR == 0) { i = c % L; l = c / L; }
+    else          { l = c % W; i = c / W; }
+    l = min(max(l, g.lo_across), min(g.lim_across, W) - 1);
+    i = min(i, min(g.lim_line, L) - 1);
+    if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
+    else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
+    else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
+}
+
+// Phase 1a: issue the loads of this thread's points (nothing waits on them here).  Every
+// lane issues every round's loads; a lane without a point reads one of the tile instead
+// (past the tile's points the last one, outside the box the nearest; the value is never
+// used).  With no lane-dependent branch around them the number of loads in flight is the
+// same on every path, so the split of round r waits for its own loads only while the
+// later rounds' (and an accumulating launch's du) loads are still in flight.
 template <int DIR, int L, int W, int K>
 __device__ __forceinline__ void tile_load(int nyg, const ws_field &d, const ws_field &u, const ws_field &v,
                                           const ws_field &w, const ws_field &cc, const TileGeo &g,
                                           TileLoads<DIR, L, W, K> &ld)
 {
     using TL = Tile<DIR, L, W, K>;
+    const int lhi = min(g.lim_across, W) - 1, phi = min(g.lim_line + 2 * LAP, TL::NL) - 1;
 #pragma unroll
     for (int r = 0; r < TL::ROUNDS; ++r) {
         int l, p, x, y, z;
-        tile_point<DIR, L, W, K>(r, l, p);
-        if (!tile_point_ok<DIR, L, W, K>(g, r, l, p)) continue;
+        tile_point_at<DIR, L, W, K>(min((int)threadIdx.x + r * TL::NT, TL::NPTS - 1), l, p);
+        l = min(max(l, g.lo_across), lhi);
+        p = min(p, phi);
         tile_point_xyz<DIR>(g, l, p, x, y, z);
         ld.v[r][0] = d.ptr[fidx(d.pitch, nyg, x, y, z)];
         ld.v[r][1] = u.ptr[fidx(u.pitch, nyg, x, y, z)];
         ld.v[r][2] = v.ptr[fidx(v.pitch, nyg, x, y, z)];
         ld.v[r][3] = w.ptr[fidx(w.pitch, nyg, x, y, z)];
         ld.v[r][4] = cc.ptr[fidx(cc.pitch, nyg, x, y, z)];
+        // rounds go out in order (the scheduler would otherwise issue them in any order,
+        // and the wait for round 0 would then cover whatever went out before it)
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -182,17 +215,12 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     const size_t vstride = (size_t)du.pitch * k.ny * k.nz;
     double duo[ACC ? CPT : 1][5];
     if (ACC) {
+        // issued by every lane, like the primitive loads, so that the split does not wait
+        // for them
 #pragma unroll
         for (int r = 0; r < CPT; ++r) {
-            const int c = t + r * TL::NT;
-            int l, i;
-            if (DIR == 0) { i = c % L; l = c / L; }
-            else          { l = c % W; i = c / W; }
-            if (c >= W * L || l < g.lo_across || l >= g.lim_across || i >= g.lim_line) continue;
             int cx, cy, cz;
-            if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
-            else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
-            else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
+            tile_cell_clamped<DIR, L, W, K>(g, t + r * TL::NT, cx, cy, cz);
             const double *dp = du.ptr + (size_t)(cx - LAP) +
                                (size_t)du.pitch * ((size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP));
 #pragma unroll
@@ -342,15 +370,8 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
         const size_t nstride = (size_t)sa.fn.pitch * k.ny * k.nz, fstride = (size_t)sa.f.pitch * k.ny * k.nz;
 #pragma unroll
         for (int r = 0; r < CPT; ++r) {
-            const int c = t + r * TL::NT;
-            int l, i;
-            if (DIR == 0) { i = c % L; l = c / L; }
-            else          { l = c % W; i = c / W; }
-            if (c >= W * L || l < g.lo_across || l >= g.lim_across || i >= g.lim_line) continue;
             int cx, cy, cz;
-            if (DIR == 0)      { cx = g.x0 + i; cy = g.y0 + l; cz = g.z0; }
-            else if (DIR == 1) { cx = g.x0 + l; cy = g.y0 + i; cz = g.z0; }
-            else               { cx = g.x0 + l; cy = g.y0; cz = g.z0 + i; }
+            tile_cell_clamped<DIR, L, W, K>(g, t + r * TL::NT, cx, cy, cz);   // every lane loads
             const size_t cell = (size_t)(cy - LAP) + (size_t)k.ny * (size_t)(cz - LAP);
             const double *np = sa.fn.ptr + (size_t)(cx - LAP) + (size_t)sa.fn.pitch * cell;
             const double *fp = sa.f.ptr + (size_t)(cx - LAP) + (size_t)sa.f.pitch * cell;
