@@ -59,10 +59,12 @@ def parse():
                          "rk3: SSP-RK3 steps of one box in a graph")
     ap.add_argument("--rk3-steps", type=int, default=0, help="RK3 steps of the batched/rk3 workloads "
                                                             "(default: 10 batched, 100 rk3)")
-    ap.add_argument("--stage-fusion", type=int, default=2, choices=[0, 1, 2],
-                    help="rk3 and batched RK3: 2 (default) lets the last RHS launch apply each stage and, "
-                         "for one box, derive the primitives of stages 1 and 2; 1 the stage only; 0 the "
-                         "separate stage kernel")
+    ap.add_argument("--stage-fusion", type=int, default=3, choices=[0, 1, 2, 3],
+                    help="rk3 and batched RK3: 3 (default) as 2 and, for one box, also stage 3's primitives, "
+                         "sound speed and CFL maxima, alternating the primitive sets from step to step (a "
+                         "batch treats 3 as 2); 2 lets the last RHS launch apply each stage and, for one box, "
+                         "derive the primitives of stages 1 and 2; 1 the stage only; 0 the separate stage "
+                         "kernel")
     ap.add_argument("--probe", action="store_true",
                     help="only measure the device's VALU rates (ws_probe_valu) and print them")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -312,6 +314,17 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
+def _ctx_choice(ctx):
+    """'shape/schedule/order' the context runs its tiled RHS with."""
+    import windstencil_amd as W
+    L = W.lib()
+    chosen, sched, order = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
+    W.check(L.ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
+    W.check(L.ws_ctx_get_direction_order(ctx.handle, ctypes.byref(order)))
+    return f"{chosen.value}/{sched.value}/{order.value}"
+
+
 def bench_rk3(args):
     """SSP-RK3 steps of one periodic box, replayed from one hipGraph per step (dt from
     the CFL reduction every step): the FP64 drop-in workload (600x128x128 vortex
@@ -367,7 +380,7 @@ def bench_rk3(args):
                    "cells": N},
                   roofline=roof, rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
                   mcell_steps_per_s=round(N * nsteps / secs / 1e6, 2),
-                  stage_fusion=args.stage_fusion,
+                  stage_fusion=args.stage_fusion, **{"tile_shape/schedule/order": _ctx_choice(ctx)},
                   totals=[float(v) for v in f.sum(dim=(1, 2, 3)).cpu()])
     out["steps"] = nsteps
     print(json.dumps(out))

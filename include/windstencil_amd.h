@@ -189,12 +189,17 @@ int ws_ctx_get_schedule(const ws_ctx *ctx, int32_t *schedule);
  * precision (the autotuner picks) */
 int ws_ctx_set_direction_order(ws_ctx *ctx, int32_t order);
 int ws_ctx_get_direction_order(const ws_ctx *ctx, int32_t *order);
-/* 2 (default): RK3 steps let the last directional RHS launch apply each stage's update
- * (tiled kernels, grid schedule, constant metrics, no closures, no viscous terms; same
- * result as the separate stage kernel, du then holds partial sums) and, with every
- * direction periodic, derive the primitives of stages 1 and 2 in that launch too (into a
- * second set of six ghosted fields the context allocates at the first eager step; same
- * state bits at the end of the step); 1: the update only; 0: separate update. */
+/* 2: RK3 steps let the last directional RHS launch apply each stage's update (tiled
+ * kernels, grid schedule, constant metrics, no closures, no viscous terms; same result as
+ * the separate stage kernel, du then holds partial sums) and, with every direction
+ * periodic, derive the primitives of stages 1 and 2 in that launch too (into a second set
+ * of six ghosted fields the context allocates at the first eager step; same state bits at
+ * the end of the step); 3 (default): as 2, and within ws_rk3_run (two steps or more) the
+ * primitives, sound speed and CFL maxima of stage 3 as well, the primitives alternating
+ * between the state's fields and the second set from step to step (no stage kernel after
+ * stage 3, no preparation before the next step; the run's last step ends in the state's
+ * fields with the bits of 2, ghost layers included; ws_rk3_step runs as 2); 1: the update
+ * only; 0: separate update.  Batches treat 3 as 2. */
 int ws_ctx_set_stage_fusion(ws_ctx *ctx, int32_t on);
 
 /* Allocate / free the solver state with the reference's extents
@@ -420,8 +425,9 @@ int ws_batch_set_tile_shape(ws_batch *batch, int32_t shape);
 int ws_batch_get_tile_shape(const ws_batch *batch, int32_t *shape);
 int ws_batch_set_schedule(ws_batch *batch, int32_t schedule);
 /* As ws_ctx_set_stage_fusion, for the batch's RK3 steps (grid schedule, no viscous terms):
- * 1 lets the batched z launch apply the stage, 2 (default) also derive the primitives of
- * stages 1 and 2 there (each block's context then holds a second primitive set). */
+ * 1 lets the batched z launch apply the stage, 2 (and 3, the default, which acts as 2 here)
+ * also derive the primitives of stages 1 and 2 there (each block's context then holds a
+ * second primitive set). */
 int ws_batch_set_stage_fusion(ws_batch *batch, int32_t on);
 int ws_batch_get_schedule(const ws_batch *batch, int32_t *schedule);
 /* RHS of every block from ghost-filled d,u,v,w and cc (overwrites du). */

@@ -172,17 +172,18 @@ def test_rk3_tiled_matches_oracle(orc, ws, kind):
     assert err <= 1e-12
 
 
-@pytest.mark.parametrize("fusion", [1, 2])
+@pytest.mark.parametrize("fusion", [1, 2, 3])
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
 @pytest.mark.parametrize("order", [W.WS_ORDER_XYZ, W.WS_ORDER_ZYX])
 @pytest.mark.parametrize("graph", [False, True])
 def test_stage_applied_by_the_last_rhs_launch_equals_the_stage_kernel(ws, precision, order, graph, fusion):
-    """RK3 steps whose last directional RHS launch applies the stage (fusion 1), and also
+    """RK3 steps whose last directional RHS launch applies the stage (fusion 1), also
     derives the primitives of stages 1 and 2 into the context's second primitive set
-    (fusion 2, the default), against the separate stage kernel reading du back: the same
-    stage and primitive expressions on the same complete RHS, so the states -- f and every
-    ghosted primitive field, ghost layers included -- agree bit for bit (boundary planes
-    on, CFL step from the device)."""
+    (fusion 2), and within the run those of stage 3 as well, alternating the primitive sets
+    from step to step with the CFL maxima taken per tile (fusion 3, the default), against
+    the separate stage kernel reading du back: the same stage and primitive expressions on
+    the same complete RHS, so the states -- f and every ghosted primitive field, ghost
+    layers included -- agree bit for bit (boundary planes on, CFL step from the device)."""
     prims = random_state(45, 26, 19, 13, "shock")
     a, sa = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=fusion)
     b, sb = _gpu_run(prims, 4, W.WS_KERNEL_TILED, graph=graph, precision=precision, order=order, fusion=0)
@@ -193,7 +194,31 @@ def test_stage_applied_by_the_last_rhs_launch_equals_the_stage_kernel(ws, precis
         assert np.array_equal(sa.t[k].cpu().numpy(), sb.t[k].cpu().numpy()), k
 
 
-@pytest.mark.parametrize("fusion", [0, 2])
+@pytest.mark.parametrize("nsteps", [1, 2, 3, 5])
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("shape", [(37, 24, 18), (64, 32, 32)])
+def test_alternating_primitive_sets_end_as_the_plain_steps(ws, precision, graph, nsteps, shape):
+    """Fusion 3 ends a run whose last step starts from the second primitive set (an even
+    number of steps) by copying that set to the state's fields before the last launch
+    writes the new primitives; an odd number of steps ends with a plain step.  Either way
+    f, the ghosted primitives (ghost layers included) and P equal those of fusion 0, for
+    runs of 1, 2, 3 and 5 steps (graph replays in between, CFL step from the device).  At
+    64x32x32 the stage kernel of a plain step has more work-groups (256) than the
+    stage-applying launch has tiles (160): a plain last step writes no CFL maxima into the
+    per-tile buffer."""
+    nx, ny, nz = shape
+    prims = random_state(nx, ny, nz, 17, "vortex")
+    log(f"fusion 3 vs 0: {nsteps} steps, precision {precision}, graph {graph}")
+    a, sa = _gpu_run(prims, nsteps, W.WS_KERNEL_TILED, graph=graph, precision=precision, fusion=3)
+    b, sb = _gpu_run(prims, nsteps, W.WS_KERNEL_TILED, graph=graph, precision=precision, fusion=0)
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b), rel_l2(a, b)
+    for k in ("d", "u", "v", "w", "T", "P", "cc"):
+        assert np.array_equal(sa.t[k].cpu().numpy(), sb.t[k].cpu().numpy()), k
+
+
+@pytest.mark.parametrize("fusion", [0, 2, 3])
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
 def test_run_cfl_from_stage_maxima_bitwise(ws, precision, fusion):
     """ws_rk3_run takes each step's dt (after the first) from the CFL maxima its last stage

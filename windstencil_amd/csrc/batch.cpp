@@ -334,7 +334,8 @@ int ws_batch_set_tile_shape(ws_batch *b, int32_t shape)
 int ws_batch_set_stage_fusion(ws_batch *b, int32_t on)
 {
     if (!b) return set_error(WS_ERR_INVALID, "null batch");
-    if (on < 0 || on > 2) return set_error(WS_ERR_INVALID, "stage fusion must be 0, 1 or 2");
+    // 3 (the contexts' default: alternating primitive sets within ws_rk3_run) acts as 2 here
+    if (on < 0 || on > 3) return set_error(WS_ERR_INVALID, "stage fusion must be 0, 1, 2 or 3");
     for (ws_ctx *c : b->ctx) c->fuse_stage = on;
     return WS_OK;
 }

@@ -283,19 +283,87 @@ double *cfl_stage_buffer(ws_ctx *c)
     return c->cfl_stage;
 }
 
-int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next)
+bool fused_stage_ok(const ws_ctx *c)
+{
+    const bool closures = c->kp.bnd_any[0] || c->kp.bnd_any[1] || c->kp.bnd_any[2];
+    return c->fuse_stage && c->cfg.kernel == WS_KERNEL_TILED && !c->cfg.viscous && c->schedule == WS_SCHED_GRID &&
+           !closures;
+}
+
+bool alternating_steps(ws_ctx *c, hipStream_t s, bool cfl)
+{
+    const bool periodic = c->cfg.periodic[0] && c->cfg.periodic[1] && c->cfg.periodic[2];
+    ws_field second[6];
+    if (c->fuse_stage < 3 || !fused_stage_ok(c) || !periodic || !prim_set(c, s, second)) return false;
+    if (!cfl) return true;
+    // room for either producer's count (tiles of the stage launch; stage-kernel work-groups)
+    const long tiles = stage_launch_tiles(*c), blocks = stage_blocks(*c);
+    const long need = tiles > blocks ? tiles : blocks;
+    if (c->cfl_tiles_n < need) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+        if (c->cfl_tiles) {
+            hipDeviceSynchronize();
+            hipFree(c->cfl_tiles);
+        }
+        if (hipMalloc(&c->cfl_tiles, sizeof(double) * need) != hipSuccess) {
+            c->cfl_tiles = nullptr;
+            c->cfl_tiles_n = 0;
+            (void)hipGetLastError();
+            return false;
+        }
+        c->cfl_tiles_n = need;
+    }
+    return true;
+}
+
+// Stage fusion 3 (modes other than STEP_PLAIN): a step from set A (the state's fields or
+// the second set, ghost-filled, sound speed included) runs stage 1 into B, stage 2 into A
+// and stage 3 into B, each stage's last RHS launch writing the primitives and sound speed of
+// its new f (and in stage 3 the CFL maxima of each tile), and one ghost fill of six fields
+// follows each stage: the next step starts from B with nothing to prepare, and the
+// stage kernel after stage 3 and the next step's preparation are gone.  The last step of a
+// run from the second set copies that set (the stage-2 primitives, ghost layers and sound
+// speed) to the state's fields and lets the stage-3 launch write the primitives and P of
+// the new state over their interior: what STEP_PLAIN leaves there (whose stage kernel
+// writes no sound speed after stage 3 and which fills no ghosts after it).
+int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next, StepMode mode,
+                   bool prepare)
 {
     std::swap(st->f, st->fn);
     const ws_soa nolap{};
     const bool periodic = c->cfg.periodic[0] && c->cfg.periodic[1] && c->cfg.periodic[2];
-    const bool closures = c->kp.bnd_any[0] || c->kp.bnd_any[1] || c->kp.bnd_any[2];
-    const bool fuse = c->fuse_stage && c->cfg.kernel == WS_KERNEL_TILED && !c->cfg.viscous &&
-                      c->schedule == WS_SCHED_GRID && !closures;
+    const bool fuse = fused_stage_ok(c);
     ws_field second[6];
     const bool prims = fuse && periodic && c->fuse_stage > 1 && prim_set(c, s, second);
     const ws_field g[6] = {st->d, st->u, st->v, st->w, st->T, st->cc};
     const bool planes = c->cfg.bc_planes != 0;
     int r = WS_OK;
+    if (mode != STEP_PLAIN) {
+        if (!prims || c->fuse_stage < 3) return set_error(WS_ERR_INVALID, "alternating RK3 step without stage fusion 3");
+        const bool from_second = mode != STEP_FROM_STATE, last = mode == STEP_LAST_FROM_SECOND;
+        const ws_field *A = from_second ? second : g, *B = from_second ? g : second;
+        if (prepare && !from_second) r = prepare_rhs_inputs(c, st, s);
+        for (int krk = 1; krk <= 3 && !r; ++krk) {
+            const ws_field *in = krk == 2 ? B : A, *out = krk == 2 ? A : B;
+            StageArgs sa{st->f, st->fn, dt_dev, c->cfg.dt, krk};
+            sa.planes = planes;
+            sa.pd = out[0]; sa.pu = out[1]; sa.pv = out[2]; sa.pw = out[3]; sa.pT = out[4];
+            if (krk == 3 && last) {
+                sa.pP = st->P;
+                r = launch_copy_fields(*c, second, g, 6, s);
+            } else {
+                sa.pcc = out[5];
+                if (krk == 3) sa.cfl_part = cfl_next;
+            }
+            if (!r)
+                r = c->cfg.precision == WS_PREC_MIXED
+                        ? launch_rhs_mixed_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s)
+                        : launch_rhs_tiled_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s);
+            if (!r && !(krk == 3 && last)) r = launch_ghost_fill(*c, out, 6, s);
+        }
+        return r;
+    }
     for (int krk = 1; krk <= 3 && !r; ++krk) {
         if (krk == 1 || !periodic) r = prepare_rhs_inputs(c, st, s);
         const bool sound = periodic && krk < 3;
@@ -407,8 +475,10 @@ int ws_ctx_destroy(ws_ctx *c)
     if (!c) return WS_OK;
     if (c->capture_stream) hipStreamDestroy(c->capture_stream);
     if (c->cfl_partial) hipFree(c->cfl_partial);
-    if (c->vis_scratch || c->prim_scratch || c->cfl_stage || c->anchor_scratch) hipDeviceSynchronize();
+    if (c->vis_scratch || c->prim_scratch || c->cfl_stage || c->cfl_tiles || c->anchor_scratch)
+        hipDeviceSynchronize();
     if (c->vis_scratch) hipFree(c->vis_scratch);
+    if (c->cfl_tiles) hipFree(c->cfl_tiles);
     if (c->anchor_scratch) hipFree(c->anchor_scratch);
     if (c->prim_scratch) hipFree(c->prim_scratch);
     if (c->cfl_stage) hipFree(c->cfl_stage);
@@ -486,7 +556,7 @@ int ws_ctx_get_direction_order(const ws_ctx *c, int32_t *order)
 int ws_ctx_set_stage_fusion(ws_ctx *c, int32_t on)
 {
     if (!c) return set_error(WS_ERR_INVALID, "null context");
-    if (on < 0 || on > 2) return set_error(WS_ERR_INVALID, "stage fusion must be 0, 1 or 2");
+    if (on < 0 || on > 3) return set_error(WS_ERR_INVALID, "stage fusion must be 0, 1, 2 or 3");
     c->fuse_stage = on;
     return WS_OK;
 }
@@ -686,27 +756,42 @@ int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_
     }
     // With a CFL step, the stage kernel that ends each step leaves the maxima of the new
     // state in c->cfl_stage, and the next step's dt is their reduction (k_cfl_final):
-    // the same maximum as a k_cfl_partial pass over the state, without that pass.
-    double *cfl_next = dt_dev ? cfl_stage_buffer(c) : nullptr;
-    // one step; the first takes its dt from a pass over the state
-    auto step = [&](ws_state *S, bool first, hipStream_t ss) {
+    // the same maximum as a k_cfl_partial pass over the state, without that pass.  With
+    // stage fusion 3 the steps alternate the primitive sets (rk3_step_fused), the
+    // stage-applying launch of each step but the last leaves the maxima per tile in
+    // c->cfl_tiles, and the last step ends in the state's fields.
+    const bool alt = nsteps > 1 && alternating_steps(c, s, dt_dev != nullptr);
+    double *cfl_next = !dt_dev ? nullptr : alt ? c->cfl_tiles : cfl_stage_buffer(c);
+    const int cfl_n = alt ? (int)stage_launch_tiles(*c) : (int)c->cfl_stage_n;
+    auto mode_of = [&](int n) {
+        if (!alt) return STEP_PLAIN;
+        if (n == nsteps - 1) return (n & 1) ? STEP_LAST_FROM_SECOND : STEP_PLAIN;
+        return (n & 1) ? STEP_FROM_SECOND : STEP_FROM_STATE;
+    };
+    // step n; the first takes its dt from a pass over the state
+    auto step = [&](ws_state *S, int n, hipStream_t ss) {
         int q = WS_OK;
         if (dt_dev)
-            q = (first || !cfl_next) ? ws_cfl_dt(c, S, cfl, dt_dev, ss)
-                                     : launch_cfl_final(*c, cfl_next, (int)c->cfl_stage_n, cfl, dt_dev, ss);
-        if (!q) q = rk3_step_fused(c, S, dt_dev, ss, cfl_next);
+            q = (n == 0 || !cfl_next) ? ws_cfl_dt(c, S, cfl, dt_dev, ss)
+                                      : launch_cfl_final(*c, cfl_next, cfl_n, cfl, dt_dev, ss);
+        // the last step's maxima would never be read (and a plain last step's stage kernel
+        // writes one per 256 cells, more than the per-tile buffer of an alternating run holds)
+        if (!q) q = rk3_step_fused(c, S, dt_dev, ss, n == nsteps - 1 ? nullptr : cfl_next, mode_of(n), n == 0);
         return q;
     };
     int r = WS_OK;
     if (!use_graph) {
-        for (int n = 0; n < nsteps && !r; ++n) r = step(st, n == 0, s);
+        for (int n = 0; n < nsteps && !r; ++n) r = step(st, n, s);
     } else {
         // Step 0 runs eagerly (it performs any lazy allocation); later steps replay one
-        // captured graph per parity of the f/fn swap, since a step ends with f and fn exchanged.
+        // captured graph per parity of the f/fn swap, since a step ends with f and fn
+        // exchanged (and, alternating, with the primitives in the other set).  An
+        // alternating run's last step, which ends differently, runs eagerly too.
         hipGraphExec_t exec[2] = {nullptr, nullptr};
-        for (int n = 0; n < nsteps && !r; ++n) {
+        const int replayed = alt ? nsteps - 1 : nsteps;
+        for (int n = 0; n < replayed && !r; ++n) {
             if (n == 0) {
-                r = step(st, true, s);
+                r = step(st, 0, s);
                 continue;
             }
             const int p = n & 1;
@@ -716,7 +801,7 @@ int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_
                 hipStream_t cs = c->capture_stream;
                 hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) { r = hip_error(e, "begin capture"); break; }
-                r = step(&probe, false, cs);
+                r = step(&probe, n, cs);
                 e = hipStreamEndCapture(cs, &g);
                 if (r) { if (g) hipGraphDestroy(g); break; }
                 if (e != hipSuccess) { r = hip_error(e, "end capture"); break; }
@@ -728,6 +813,7 @@ int ws_rk3_run(ws_ctx *c, ws_state *st, int32_t nsteps, double cfl, int32_t use_
             if (e != hipSuccess) { r = hip_error(e, "graph launch"); break; }
             std::swap(st->f, st->fn);
         }
+        if (!r && alt) r = step(st, nsteps - 1, s);
         for (int p = 0; p < 2; ++p)
             if (exec[p]) {
                 hipStreamSynchronize(s);

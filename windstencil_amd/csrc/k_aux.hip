@@ -61,6 +61,18 @@ __global__ void __launch_bounds__(NT) k_ghost_fill(FieldList fl, int nx, int ny,
                 (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// to = from over a ghosted box of nxg x rows points (rows = nyg * nzg), one field per
+// blockIdx.y; each field's pitch is at least nxg
+__global__ void __launch_bounds__(NT) k_copy_fields(FieldList from, FieldList to, int nxg, long rows)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= (long)nxg * rows) return;
+    const long row = i / nxg;
+    const int x = (int)(i - row * nxg);
+    const ws_field &a = from.f[blockIdx.y], &b = to.f[blockIdx.y];
+    b.ptr[(size_t)x + (size_t)b.pitch * (size_t)row] = a.ptr[(size_t)x + (size_t)a.pitch * (size_t)row];
+}
+
 __global__ void __launch_bounds__(NT) k_sound_speed(ws_field T, ws_field cc, int sx, int sy, int sz, int nx,
                                                     int ny, int nz, int nyg, double ama)
 {
@@ -411,6 +423,24 @@ int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s)
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? WS_OK : hip_error(e, "ghost fill launch");
+}
+
+int launch_copy_fields(const Ctx &c, const ws_field *from, const ws_field *to, int n, hipStream_t s)
+{
+    const KParams &k = c.kp;
+    const int nxg = k.nx + 2 * LAP;
+    const long rows = (long)k.nyg * k.nzg;
+    if (n < 1 || n > 8) return set_error(WS_ERR_INVALID, "copy of 1 to 8 fields");
+    FieldList a{}, b{};
+    for (int q = 0; q < n; ++q) {
+        if (from[q].pitch < (uint32_t)nxg || to[q].pitch < (uint32_t)nxg)
+            return set_error(WS_ERR_INVALID, "field copy: pitch below the ghosted row");
+        a.f[q] = from[q];
+        b.f[q] = to[q];
+    }
+    hipLaunchKernelGGL(k_copy_fields, dim3(blocks_for((long)nxg * rows), n), dim3(NT), 0, s, a, b, nxg, rows);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? WS_OK : hip_error(e, "field copy launch");
 }
 
 int launch_prepare(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T, ws_field cc,

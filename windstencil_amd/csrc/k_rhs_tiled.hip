@@ -16,6 +16,20 @@ int launch_rhs_tiled_stage(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_
     return launch_rhs_tile<false>(c, d, u, v, w, cc, nullptr, du, job, 0, s, &sa);
 }
 
+long stage_launch_tiles(const Ctx &c)
+{
+    Box box;
+    for (int d = 0; d < 3; ++d) box.s[d] = LAP;
+    box.e[0] = c.kp.nx + LAP; box.e[1] = c.kp.ny + LAP; box.e[2] = c.kp.nz + LAP;
+    switch (c.tile_shape) {
+    case 0: return stage_tiles_n<0>(box, c.dir_order);
+    case 1: return stage_tiles_n<1>(box, c.dir_order);
+    case 2: return stage_tiles_n<2>(box, c.dir_order);
+    case 3: return stage_tiles_n<3>(box, c.dir_order);
+    default: return stage_tiles_n<4>(box, c.dir_order);
+    }
+}
+
 int launch_rhs_batch_tiled(const KParams &k, const BatchTiles &bt, hipStream_t s)
 {
     return launch_rhs_batch_tile<false>(k, bt, s);

@@ -196,10 +196,12 @@ __device__ __forceinline__ void tile_load(int nyg, const ws_field &d, const ws_f
 // write the flux difference of every cell.  Every thread of the block calls it.
 // With STG the launch is the last of the three and applies the RK3 stage of `sa` to its
 // cells: f = stage(fn, f, du + this direction's terms), du itself is not written.
+// Returns, when the launch applies a stage with sa.cfl_part set, the maximum CFL speed of
+// this thread's cells (0 otherwise).
 template <int DIR, int L, int W, int K, bool FIELDS, bool ACC, bool MIX, bool BND, bool STG, typename Next>
-__device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_fields &mf, const ws_soa &du,
-                                             const TileGeo &g, TileLoads<DIR, L, W, K> &ld, double *sp,
-                                             const StageArgs &sa, Next next)
+__device__ __forceinline__ double tile_compute(const KParams &k, const ws_metric_fields &mf, const ws_soa &du,
+                                               const TileGeo &g, TileLoads<DIR, L, W, K> &ld, double *sp,
+                                               const StageArgs &sa, Next next)
 {
     static_assert(!(STG && (BND || !ACC)), "the stage is applied by an accumulating launch without closures");
     using TL = Tile<DIR, L, W, K>;
@@ -401,6 +403,7 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
     // ---- phase 3: flux difference of each cell ---------------------------------------------
     const double scale0 = -k.rh[DIR];
     const double sdt = STG ? (sa.dt_dev ? *sa.dt_dev : sa.dt) : 0.0;
+    double cfl_max = 0.0;
 #pragma unroll
     for (int r = 0; r < CPT; ++r) {
         const int c = t + r * TL::NT;
@@ -461,7 +464,11 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
                 sa.pv.ptr[fidx(sa.pv.pitch, k.nyg, cx, cy, cz)] = p.v;
                 sa.pw.ptr[fidx(sa.pw.pitch, k.nyg, cx, cy, cz)] = p.w;
                 sa.pT.ptr[fidx(sa.pT.pitch, k.nyg, cx, cy, cz)] = p.T;
-                sa.pcc.ptr[fidx(sa.pcc.pitch, k.nyg, cx, cy, cz)] = sqrt(p.T) / k.ama;
+                // the stage kernel's expressions for the sound speed, pressure and CFL speed
+                const double c = sqrt(p.T) / k.ama;
+                if (sa.pcc.ptr) sa.pcc.ptr[fidx(sa.pcc.pitch, k.nyg, cx, cy, cz)] = c;
+                if (sa.pP.ptr) sa.pP.ptr[fidx(sa.pP.pitch, k.nyg, cx, cy, cz)] = p.e / p.d * (k.gamma - 1.0);
+                if (sa.cfl_part) cfl_max = fmax(cfl_max, fmax(fabs(p.u) + c, fmax(fabs(p.v) + c, fabs(p.w) + c)));
                 const int ix = cx - LAP, iy = cy - LAP, iz = cz - LAP;
                 if (sa.planes && (ix == 0 || ix == k.nx - 1 || iy == 0 || iy == k.ny - 1 || iz == 0 || iz == k.nz - 1))
                     plane_conserved(p, k.cv, rv);
@@ -472,6 +479,27 @@ __device__ __forceinline__ void tile_compute(const KParams &k, const ws_metric_f
 #pragma unroll
             for (int m = 0; m < 5; ++m) sa.f.ptr[fo + m * fstride] = rv[m];
         }
+    }
+    return cfl_max;
+}
+
+// *out = the maximum of v over the work-group's NTH threads and 0 (every thread calls it;
+// red: NTH doubles of LDS nobody reads meanwhile).  Through LDS rather than lane shuffles:
+// a work-group of 255 threads has an inactive lane whose registers a shuffle would read.
+// Wavefront 0 is full (NTH >= 64) and finishes the reduction; fmax is exact and ignores
+// NaN, so the order does not matter.
+template <int NTH>
+__device__ __forceinline__ void tile_wg_max(double v, double *red, double *out)
+{
+    static_assert(NTH >= 64, "the reduction needs a full first wavefront");
+    red[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double b = 0.0;
+        for (int i = threadIdx.x; i < NTH; i += 64) b = fmax(b, red[i]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) b = fmax(b, __shfl_xor(b, o));
+        if (threadIdx.x == 0) *out = b;
     }
 }
 
@@ -494,7 +522,11 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
     const TileGeo g = tile_geo<DIR, L, W, K>(box, tile - r * gx, r % gy, r / gy);
     TileLoads<DIR, L, W, K> ld;
     tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
-    tile_compute<DIR, L, W, K, FIELDS, ACC, MIX, BND, STG>(k, mf, du, g, ld, sp, sa, [] {});
+    const double cmax = tile_compute<DIR, L, W, K, FIELDS, ACC, MIX, BND, STG>(k, mf, du, g, ld, sp, sa, [] {});
+    if (STG && sa.cfl_part) {
+        __syncthreads();   // phase 3 has read its faces from sp
+        tile_wg_max<TL::NT>(cmax, sp, sa.cfl_part + tile);
+    }
 }
 
 // Persistent form for one box: the resident work-groups each walk a contiguous run of
@@ -683,6 +715,18 @@ void launch_dir(const KParams &k, ws_field d, ws_field u, ws_field v, ws_field w
     }
 }
 
+// Tile grid of a grid-schedule launch over `box` (x tiles fastest).
+template <int DIR, int L, int W, int K>
+dim3 tile_grid(const Box &box)
+{
+    using TL = Tile<DIR, L, W, K>;
+    const int nx = box.e[0] - box.s[0], ny = box.e[1] - box.s[1], nz = box.e[2] - box.s[2];
+    const int nxa = box.e[0] - box.s[0] / TL::XALIGN * TL::XALIGN;
+    if (DIR == 0) return dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
+    if (DIR == 1) return dim3((nxa + W - 1) / W, (ny + L - 1) / L, nz);
+    return dim3((nxa + W - 1) / W, (nz + L - 1) / L, ny);
+}
+
 // The last launch of an RK3 stage that applies the stage itself (grid schedule, constant
 // metrics, no closures).
 template <int DIR, int L, int W, int K, bool MIX>
@@ -690,12 +734,7 @@ void launch_dir_stage(const KParams &k, ws_field d, ws_field u, ws_field v, ws_f
                       const Box &box, const StageArgs &sa, hipStream_t s)
 {
     using TL = Tile<DIR, L, W, K>;
-    const int nx = box.e[0] - box.s[0], ny = box.e[1] - box.s[1], nz = box.e[2] - box.s[2];
-    const int nxa = box.e[0] - box.s[0] / TL::XALIGN * TL::XALIGN;
-    dim3 grid;
-    if (DIR == 0) grid = dim3((nx + L - 1) / L, (ny + W - 1) / W, nz);
-    else if (DIR == 1) grid = dim3((nxa + W - 1) / W, (ny + L - 1) / L, nz);
-    else grid = dim3((nxa + W - 1) / W, (nz + L - 1) / L, ny);
+    const dim3 grid = tile_grid<DIR, L, W, K>(box);
     const int ntiles = (int)(grid.x * grid.y * grid.z);
     const ws_metric_fields nomf{};
     hipLaunchKernelGGL((k_rhs_tile<DIR, L, W, K, false, true, MIX, false, true>), dim3(8 * ((ntiles + 7) / 8)),
@@ -758,6 +797,15 @@ void launch_shape(int shape, const KParams &k, ws_field d, ws_field u, ws_field 
     case 3: launch_shape_n<3, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, sa, s); break;
     default: launch_shape_n<4, FIELDS, MIX>(k, d, u, v, w, cc, mf, du, box, acc, sched, order, sa, s); break;
     }
+}
+
+// tiles of the stage-applying launch (the last direction of `order`) of tile shape S
+template <int S>
+long stage_tiles_n(const Box &box, int order)
+{
+    constexpr TileShapeDef D = kTileShapes[S];
+    const dim3 g = order == WS_ORDER_ZYX ? tile_grid<0, D.L0, D.W0, D.K>(box) : tile_grid<2, D.L2, D.W2, D.K>(box);
+    return (long)g.x * g.y * g.z;
 }
 
 // ---- batched launches ------------------------------------------------------------------

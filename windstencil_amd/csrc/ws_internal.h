@@ -42,17 +42,22 @@ struct KParams {
 // The SSP-RK3 stage the last directional RHS launch applies to its cells itself (f from fn,
 // f and the complete RHS, which then never goes through du): stage krk (1..3), dt from
 // dt_dev when that is set.  With pd.ptr set the launch also writes the primitives d, u, v,
-// w, T and the sound speed of the new f to the interior cells of pd, pu, pv, pw, pT, pcc
-// (fields that no launch of the same RHS reads) and, with planes, re-derives f on the
-// boundary planes from them.
+// w, T of the new f to the interior cells of pd, pu, pv, pw, pT (fields that no launch of
+// the same RHS reads), the sound speed to pcc and the pressure to pP where those are set,
+// and, with planes, re-derives f on the boundary planes from them.  With cfl_part set
+// (and pd) each tile of the launch also leaves the maximum over its cells of the CFL
+// speed max(|u|, |v|, |w|) + c of the new state at cfl_part[tile] (stage_launch_tiles
+// entries; the expression of k_cfl_partial).
 struct StageArgs {
     ws_soa f, fn;
     const double *dt_dev;
     double dt;
     int krk;
     int planes = 0;
-    ws_field pd{}, pu{}, pv{}, pw{}, pT{}, pcc{};
+    ws_field pd{}, pu{}, pv{}, pw{}, pT{}, pcc{}, pP{};
+    double *cfl_part = nullptr;
 };
+
 
 struct Ctx {
     ws_config cfg;
@@ -70,9 +75,10 @@ struct Ctx {
     int schedule = WS_SCHED_GRID;
     // order of the directional launches (WS_ORDER_*)
     int dir_order = WS_ORDER_XYZ;
-    // RK3 steps apply each stage's update in the last RHS launch where they can (1), and
-    // there also derive the primitives of stages 1 and 2 (2)
-    int fuse_stage = 2;
+    // RK3 steps apply each stage's update in the last RHS launch where they can (1), there
+    // also derive the primitives of stages 1 and 2 (2), and within ws_rk3_run those of
+    // stage 3 too, alternating the primitive sets from step to step (3)
+    int fuse_stage = 3;
     // viscous terms of the two-kernel form (anchor family, metric fields): twelve ghosted
     // flux fields, allocated at the first use; planes per work-group of the fused form
     double *vis_scratch = nullptr;
@@ -86,6 +92,9 @@ struct Ctx {
     // (one per stage-kernel work-group), allocated by the first eager step
     double *cfl_stage = nullptr;
     long cfl_stage_n = 0;
+    // the same maxima per tile of the stage-applying RHS launch (stage fusion 3)
+    double *cfl_tiles = nullptr;
+    long cfl_tiles_n = 0;
     // split fluxes of the anchor kernels (10 doubles per point of the largest directional
     // sweep of the interior), allocated by the context's first anchor RHS
     double *anchor_scratch = nullptr;
@@ -180,6 +189,9 @@ int launch_rhs_tiled(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field 
                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
 int launch_rhs_mixed(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field cc,
                      const ws_metric_fields *m, ws_soa du, ws_job job, int accumulate, hipStream_t s);
+// tiles of the context's launch that applies an RK3 stage (the last direction of its order,
+// in its tile shape): the entries that launch writes to StageArgs::cfl_part
+long stage_launch_tiles(const Ctx &c);
 // the RHS with the RK3 stage `sa` applied by its last launch (grid schedule; the caller
 // checks that the configuration allows it: tiled kernels, constant metrics, no closures,
 // no viscous terms); du holds partial sums afterwards
@@ -200,9 +212,21 @@ int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s);
 // du_comput on inputs already prepared (ghosted primitives and sound speed): the inviscid
 // RHS, plus the viscous terms when the configuration has them
 int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s);
+// How a step of stage fusion 3 treats the primitive sets (rk3_step_fused): STEP_PLAIN starts
+// and ends with the primitives in the state's fields (every level); STEP_FROM_STATE and
+// STEP_FROM_SECOND start from the state's fields or the context's second set, prepared, and
+// end in the other one, prepared for the next step; STEP_LAST_FROM_SECOND starts from the
+// second set and ends in the state's fields as STEP_PLAIN leaves them.
+enum StepMode { STEP_PLAIN = 0, STEP_FROM_STATE, STEP_FROM_SECOND, STEP_LAST_FROM_SECOND };
 // one SSP-RK3 step of st with fused stage kernels (the same arithmetic as du_comput,
-// time_advance, cons_to_pri and bc_planes in turn)
-int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next = nullptr);
+// time_advance, cons_to_pri and bc_planes in turn).  cfl_next: where the step leaves the
+// CFL maxima of the new state (cfl_stage for STEP_PLAIN, cfl_tiles for the other modes).
+// prepare: STEP_FROM_STATE prepares the state's fields first (the first step of a run).
+int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s, double *cfl_next = nullptr,
+                   StepMode mode = STEP_PLAIN, bool prepare = true);
+// whether ws_rk3_run can alternate the primitive sets (stage fusion 3 and every condition of
+// the primitives-deriving fusion), allocating what that needs (outside a stream capture)
+bool alternating_steps(ws_ctx *c, hipStream_t s, bool cfl);
 // the context's second set of ghosted primitive fields (d, u, v, w, T, cc), allocated at
 // the first call outside a stream capture; 0 if it is not available
 int prim_set(ws_ctx *c, hipStream_t s, ws_field out[6]);
@@ -223,6 +247,8 @@ int launch_comput_jacobian(const Ctx &c, const ws_field *coords, const ws_metric
 int launch_prepare(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w, ws_field T, ws_field cc,
                    hipStream_t s);
 int launch_ghost_fill(const Ctx &c, const ws_field *f, int n, hipStream_t s);
+// to[q] = from[q] over the whole ghosted box (ghost layers included), q < n <= 8
+int launch_copy_fields(const Ctx &c, const ws_field *from, const ws_field *to, int n, hipStream_t s);
 int launch_cons_to_pri(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w,
                        ws_field T, ws_field P, hipStream_t s);
 int launch_pri_to_cons(const Ctx &c, ws_soa f, ws_field d, ws_field u, ws_field v, ws_field w,
