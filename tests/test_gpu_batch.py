@@ -173,7 +173,40 @@ def test_batch_stage_fusion_levels(ws, precision, graph, cfl):
         assert err <= (1e-14 if precision == W.WS_PREC_FP64 else 1e-9), (i, err)
 
 
-@pytest.mark.parametrize("fusion", [0, 2])
+@pytest.mark.parametrize("nsteps", [2, 3, 4])
+@pytest.mark.parametrize("cfl", [-1.0, 0.5])
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_alternating_primitive_sets_equal_level_2(ws, graph, cfl, nsteps):
+    """Stage fusion 3 in a batch: the z launch of stage 3 also writes every block's
+    primitives and sound speed, with the CFL maxima per tile (reduced per block over the
+    block's range of the tile list), and the primitive sets alternate from step to step; a
+    run's last step ends in the blocks' fields (from the second sets: after copying them
+    there).  The same kernels as level 2, so f, every ghosted primitive field and P agree
+    bit for bit with level 2 after 2, 3 and 4 steps (both last-step forms), eager and
+    graph, with a fixed dt and with the CFL step."""
+    extents = _extents(29, 5, 6, 30)
+    kinds = [k for k in KINDS if k != "rough"]
+    runs = []
+    for level in (2, 3):
+        b, _ = _batch(extents, 710, kinds=kinds, bc_planes=1, dt=2e-4)
+        W.check(W.lib().ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
+        W.check(W.lib().ws_batch_set_stage_fusion(b.handle, level))
+        for e, st in zip(extents, b.states):
+            _to_cons(make_ctx(*e), st)
+        log(f"batch fusion {level} vs 2: {nsteps} steps, graph {graph}, cfl {cfl}")
+        W.check(W.lib().ws_batch_rk3_run(b.handle, nsteps, cfl, graph, W.stream_handle()))
+        b.sync_states()
+        _sync()
+        runs.append([{k: st.t[k].cpu().numpy() for k in ("f", "d", "u", "v", "w", "T", "cc", "P")}
+                     for st in b.states])
+        b.close()
+    for i, (a, c) in enumerate(zip(*runs)):
+        assert np.isfinite(c["f"]).all(), i
+        for k in a:
+            assert np.array_equal(a[k], c[k]), (i, k)
+
+
+@pytest.mark.parametrize("fusion", [0, 2, 3])
 @pytest.mark.parametrize("graph", [0, 1])
 def test_batch_run_cfl_from_stage_maxima_bitwise(ws, graph, fusion):
     """ws_batch_rk3_run takes each step's dt (after the first) from the CFL maxima the

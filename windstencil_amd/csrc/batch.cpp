@@ -49,6 +49,10 @@ struct ws_batch {
     long ghost_wgs = 0, cell_wgs = 0;
     double *d_cfl_part = nullptr;   // the CFL maxima of the batched stage kernel (cell_wgs)
     bool cfl_in_batch = false;      // the last step's stage kernel wrote them there
+    // per tile shape: each block's range of the z tile list (the stage-applying launch's)
+    int2 *d_tile_ranges[WS_TILE_COUNT] = {};
+    // the CFL maxima per tile of stage fusion 3's stage-3 launch (the longest z list)
+    double *d_cfl_tiles = nullptr;
     hipStream_t capture_stream = nullptr;
     long long cells = 0;
 };
@@ -240,6 +244,7 @@ int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *exten
         BatchBlock &h = host[i];
         std::memset(&h, 0, sizeof(h));
         h.d = states[i].d; h.u = states[i].u; h.v = states[i].v; h.w = states[i].w; h.cc = states[i].cc;
+        h.P = states[i].P;
         h.du = states[i].du;
         h.fa = states[i].f;
         h.fb = states[i].fn;
@@ -258,7 +263,18 @@ int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *exten
     for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
         for (int dir = 0; dir < 3; ++dir) {
             std::vector<int4> list;
-            for (int i = 0; i < nblocks; ++i) block_tiles(i, host[i].n, shape, dir, list);
+            std::vector<int2> ranges(nblocks);
+            for (int i = 0; i < nblocks; ++i) {
+                ranges[i].x = (int)list.size();
+                block_tiles(i, host[i].n, shape, dir, list);
+                ranges[i].y = (int)list.size();
+            }
+            if (dir == 2) {
+                e = hipMalloc(&b->d_tile_ranges[shape], sizeof(int2) * nblocks);
+                if (e != hipSuccess) return fail(b, set_error(WS_ERR_NOMEM, "batch tile ranges"));
+                e = hipMemcpy(b->d_tile_ranges[shape], ranges.data(), sizeof(int2) * nblocks, hipMemcpyHostToDevice);
+                if (e != hipSuccess) return fail(b, hip_error(e, "batch tile ranges upload"));
+            }
             if (list.size() > (size_t)0x7fffffff) return fail(b, set_error(WS_ERR_INVALID, "too many tiles"));
             b->ntiles[shape][dir] = (int)list.size();
             e = hipMalloc(&b->d_tiles[shape][dir], sizeof(int4) * list.size());
@@ -291,6 +307,9 @@ int ws_batch_destroy(ws_batch *b)
     if (b->d_dt) hipFree(b->d_dt);
     if (b->d_aux) hipFree(b->d_aux);
     if (b->d_cfl_part) hipFree(b->d_cfl_part);
+    for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
+        if (b->d_tile_ranges[shape]) hipFree(b->d_tile_ranges[shape]);
+    if (b->d_cfl_tiles) hipFree(b->d_cfl_tiles);
     if (b->capture_stream) hipStreamDestroy(b->capture_stream);
     delete b;
     return WS_OK;
@@ -392,9 +411,16 @@ int ws_batch_cfl_dt(ws_batch *b, double cfl, double *dt_dev, hipStream_t s)
     return launch_min_reduce(b->d_dts, (int)b->ctx.size(), dt_dev, s);
 }
 
-// One batched RK3 step; cfl_next: the per-block stage kernels that end it also leave the
-// blocks' CFL maxima for the next step's dt (each block's cfl_stage_buffer).
-static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl_next)
+// One batched RK3 step; cfl_next: the step leaves the blocks' CFL maxima for the next
+// step's dt (STEP_PLAIN: its stage kernels, in the batch's buffer or each block's
+// cfl_stage_buffer; the alternating modes: its stage-3 launch, per tile in d_cfl_tiles).
+// mode and prepare as for a single box's rk3_step_fused (stage fusion 3): an alternating
+// step from set A runs stage 1 into B, stage 2 into A and stage 3 into B with the batched
+// z launch writing every block's primitives and sound speed, one batched ghost fill after
+// each stage; the last step of a run from the second sets copies them to the blocks'
+// fields and lets the stage-3 launch write the primitives and P over their interior.
+static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl_next, StepMode mode = STEP_PLAIN,
+                      bool prepare = true)
 {
     if (!b) return set_error(WS_ERR_INVALID, "null batch");
     for (ws_state &st : b->st) std::swap(st.f, st.fn);
@@ -419,6 +445,39 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
     const int nb = (int)b->ctx.size();
     const bool planes = b->base.bc_planes != 0;
     b->cfl_in_batch = false;
+    if (mode != STEP_PLAIN) {
+        if (!(fuse && prims && aux) || b->ctx[0]->fuse_stage < 3 || b->kernel != WS_KERNEL_TILED)
+            return set_error(WS_ERR_INVALID, "alternating batched RK3 step without stage fusion 3");
+        const bool from_second = mode != STEP_FROM_STATE, last = mode == STEP_LAST_FROM_SECOND;
+        // d_blocks reads the blocks' fields and writes their second sets, d_blocks2 the reverse
+        const BatchBlock *toB = from_second ? b->d_blocks2 : b->d_blocks;
+        const BatchBlock *toA = from_second ? b->d_blocks : b->d_blocks2;
+        const int setA = from_second ? 1 : 0, setB = 1 - setA;
+        if (prepare && !from_second)
+            r = launch_prepare_batch(b->ctx[0]->kp, b->d_aux, nb, b->ghost_wgs, b->cell_wgs, s);
+        for (int krk = 1; krk <= 3 && !r; ++krk) {
+            BatchStage bs{dt_dev, b->base.dt, krk, swapped};
+            bs.prims = 1;
+            bs.planes = planes;
+            if (krk == 3 && last) {
+                bs.cc = 0;
+                bs.pressure = 1;
+                for (size_t i = 0; i < b->ctx.size() && !r; ++i) {
+                    const BatchBlock &h = b->host_blocks[i];
+                    const ws_state &st = b->st[i];
+                    const ws_field sec[6] = {h.od, h.ou, h.ov, h.ow, h.oT, h.occ};
+                    const ws_field own[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
+                    r = launch_copy_fields(*b->ctx[i], sec, own, 6, s);
+                }
+            } else if (krk == 3 && cfl_next) {
+                bs.cfl_part = b->d_cfl_tiles;
+            }
+            if (!r) r = batch_rhs(b, s, &bs, krk == 2 ? toA : toB);
+            if (!r && !(krk == 3 && last))
+                r = launch_ghost_fill_batch(b->d_aux, nb, b->ghost_wgs, krk == 2 ? setA : setB, s);
+        }
+        return r;
+    }
     // the blocks are periodic (ws_batch_create); stages 1 and 2 leave the next stage's
     // inputs prepared (fused stage kernel writing the sound speed, then one ghost fill)
     for (int krk = 1; krk <= 3 && !r; ++krk) {
@@ -475,6 +534,27 @@ int ws_batch_rk3_step(ws_batch *b, const double *dt_dev, hipStream_t s)
     return batch_step(b, dt_dev, s, false);
 }
 
+// Whether ws_batch_rk3_run can alternate the primitive sets (stage fusion 3 and every
+// condition of the batched primitives-deriving fusion), allocating what that needs.
+static bool batch_alternating(ws_batch *b, hipStream_t s, bool cfl)
+{
+    if (b->kernel != WS_KERNEL_TILED || b->base.viscous || b->schedule != WS_SCHED_GRID ||
+        b->ctx[0]->fuse_stage < 3)
+        return false;
+    if (!prim_tables(b, s) || !aux_table(b, s) || !b->aux_second) return false;
+    if (!cfl || b->d_cfl_tiles) return true;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+    long n = 0;   // room for the z tile list of every shape
+    for (int shape = 0; shape < WS_TILE_COUNT; ++shape) n = std::max(n, (long)b->ntiles[shape][2]);
+    if (hipMalloc(&b->d_cfl_tiles, sizeof(double) * n) != hipSuccess) {
+        b->d_cfl_tiles = nullptr;
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
 int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t s)
 {
     if (!b || nsteps < 0) return set_error(WS_ERR_INVALID, "bad RK3 request");
@@ -484,32 +564,49 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
     // minimum over the blocks; the same dt bits as ws_batch_cfl_dt.
     bool staged = dt_dev != nullptr;
     for (ws_ctx *c : b->ctx) staged = staged && cfl_stage_buffer(c);
-    auto cfl_dt = [&](bool first, hipStream_t ss) {
-        if (first || !staged) return ws_batch_cfl_dt(b, cfl, dt_dev, ss);
-        // the last step's stage kernel left its maxima in the batch's buffer
-        if (b->cfl_in_batch)
-            return launch_cfl_batch(b->d_aux, (int)b->ctx.size(), b->d_cfl_part, cfl, b->d_dts, dt_dev, ss);
+    // Stage fusion 3: the steps alternate the primitive sets (batch_step), the stage-3
+    // launch of each step but the last leaves the maxima per tile, and the last step ends
+    // in the blocks' fields.
+    const bool alt = nsteps > 1 && batch_alternating(b, s, dt_dev != nullptr);
+    auto mode_of = [&](int n) {
+        if (!alt) return STEP_PLAIN;
+        if (n == nsteps - 1) return (n & 1) ? STEP_LAST_FROM_SECOND : STEP_PLAIN;
+        return (n & 1) ? STEP_FROM_SECOND : STEP_FROM_STATE;
+    };
+    const int nb = (int)b->ctx.size();
+    auto cfl_dt = [&](int n, hipStream_t ss) {
+        if (n == 0 || !staged) return ws_batch_cfl_dt(b, cfl, dt_dev, ss);
+        // the previous step left its maxima per tile (alternating) or in the batch's buffer
+        if (alt)
+            return launch_cfl_batch(b->d_aux, nb, b->d_cfl_tiles, cfl, b->d_dts, dt_dev, ss,
+                                    b->d_tile_ranges[b->tile_shape]);
+        if (b->cfl_in_batch) return launch_cfl_batch(b->d_aux, nb, b->d_cfl_part, cfl, b->d_dts, dt_dev, ss);
         for (size_t i = 0; i < b->ctx.size(); ++i) {
             ws_ctx *c = b->ctx[i];
             const int q = launch_cfl_final(*c, c->cfl_stage, (int)c->cfl_stage_n, cfl, b->d_dts + i, ss);
             if (q) return q;
         }
-        return launch_min_reduce(b->d_dts, (int)b->ctx.size(), dt_dev, ss);
+        return launch_min_reduce(b->d_dts, nb, dt_dev, ss);
+    };
+    // the last step's maxima would never be read
+    auto step = [&](int n, hipStream_t ss) {
+        int q = WS_OK;
+        if (dt_dev) q = cfl_dt(n, ss);
+        if (!q) q = batch_step(b, dt_dev, ss, staged && n < nsteps - 1, mode_of(n), n == 0);
+        return q;
     };
     int r = WS_OK;
     if (!use_graph) {
-        for (int n = 0; n < nsteps && !r; ++n) {
-            if (dt_dev) r = cfl_dt(n == 0, s);
-            if (!r) r = batch_step(b, dt_dev, s, staged);
-        }
+        for (int n = 0; n < nsteps && !r; ++n) r = step(n, s);
     } else {
-        // one captured graph per parity of the f/fn swap (a step ends with them exchanged);
-        // step 0 runs eagerly
+        // one captured graph per parity of the f/fn swap (a step ends with them exchanged,
+        // and, alternating, with the primitives in the other set); step 0 runs eagerly, and
+        // so does an alternating run's last step, which ends differently
         hipGraphExec_t exec[2] = {nullptr, nullptr};
-        for (int n = 0; n < nsteps && !r; ++n) {
+        const int replayed = alt ? nsteps - 1 : nsteps;
+        for (int n = 0; n < replayed && !r; ++n) {
             if (n == 0) {
-                if (dt_dev) r = cfl_dt(true, s);
-                if (!r) r = batch_step(b, dt_dev, s, staged);
+                r = step(0, s);
                 continue;
             }
             const int p = n & 1;
@@ -518,8 +615,7 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
                 if (e != hipSuccess) { r = hip_error(e, "begin capture"); break; }
-                if (dt_dev) r = cfl_dt(false, b->capture_stream);
-                if (!r) r = batch_step(b, dt_dev, b->capture_stream, staged);
+                r = step(n, b->capture_stream);
                 e = hipStreamEndCapture(b->capture_stream, &g);
                 b->st = saved;   // capture recorded the step; the swap happens at replay
                 if (r) { if (g) hipGraphDestroy(g); break; }
@@ -532,6 +628,7 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
             if (e != hipSuccess) { r = hip_error(e, "graph launch"); break; }
             for (ws_state &st : b->st) std::swap(st.f, st.fn);
         }
+        if (!r && alt) r = step(nsteps - 1, s);
         for (int p = 0; p < 2; ++p)
             if (exec[p]) {
                 hipStreamSynchronize(s);

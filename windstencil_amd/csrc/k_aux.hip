@@ -387,13 +387,15 @@ __global__ void __launch_bounds__(NT) k_stage_batch(KParams k, const AuxBlock *t
 
 // one work-group per block: k_cfl_final over the block's partials
 __global__ void __launch_bounds__(NT) k_cfl_final_batch(const AuxBlock *tab, const double *part, double cfl,
-                                                        double *dts)
+                                                        double *dts, const int2 *ranges)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
     const AuxBlock &b = tab[blockIdx.x];
+    const long lo = ranges ? (long)ranges[blockIdx.x].x : b.cell_wg0;
+    const long hi = ranges ? (long)ranges[blockIdx.x].y : b.cell_wg1;
     double mx = 0.0;
-    for (long i = b.cell_wg0 + threadIdx.x; i < b.cell_wg1; i += NT) mx = fmax(mx, part[i]);
+    for (long i = lo + threadIdx.x; i < hi; i += NT) mx = fmax(mx, part[i]);
     red[threadIdx.x] = mx;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
@@ -631,10 +633,10 @@ int launch_stage_batch(const KParams &k, const AuxBlock *tab, int nb, long cell_
 }
 
 int launch_cfl_batch(const AuxBlock *tab, int nb, const double *cfl_part, double cfl, double *dts, double *dt_dev,
-                     hipStream_t s)
+                     hipStream_t s, const int2 *ranges)
 {
     if (nb <= 0) return set_error(WS_ERR_INVALID, "empty batch");
-    hipLaunchKernelGGL(k_cfl_final_batch, dim3(nb), dim3(NT), 0, s, tab, cfl_part, cfl, dts);
+    hipLaunchKernelGGL(k_cfl_final_batch, dim3(nb), dim3(NT), 0, s, tab, cfl_part, cfl, dts, ranges);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "batched cfl launch");
     return launch_min_reduce(dts, nb, dt_dev, s);

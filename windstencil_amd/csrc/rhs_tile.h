@@ -654,10 +654,17 @@ k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int nti
         sa = StageArgs{bs.swapped ? b.fb : b.fa, bs.swapped ? b.fa : b.fb, bs.dt_dev, bs.dt, bs.krk};
         if (bs.prims) {
             sa.planes = bs.planes;
-            sa.pd = b.od; sa.pu = b.ou; sa.pv = b.ov; sa.pw = b.ow; sa.pT = b.oT; sa.pcc = b.occ;
+            sa.pd = b.od; sa.pu = b.ou; sa.pv = b.ov; sa.pw = b.ow; sa.pT = b.oT;
+            if (bs.cc) sa.pcc = b.occ;
+            if (bs.pressure) sa.pP = b.P;
+            sa.cfl_part = bs.cfl_part;
         }
     }
-    tile_compute<DIR, L, W, K, false, ACC, MIX, false, STG>(kb, nomf, b.du, g, ld, sp, sa, [] {});
+    const double cmax = tile_compute<DIR, L, W, K, false, ACC, MIX, false, STG>(kb, nomf, b.du, g, ld, sp, sa, [] {});
+    if (STG && sa.cfl_part) {
+        __syncthreads();   // phase 3 has read its faces from sp
+        tile_wg_max<TL::NT>(cmax, sp, sa.cfl_part + tile);
+    }
 }
 
 // work-groups of a persistent kernel that fit on the device at once

@@ -111,6 +111,7 @@ struct BatchBlock {
     // where a stage-applying launch with BatchStage::prims writes the new primitives
     // (d, u, v, w, T, cc): the other primitive set of the block
     ws_field od, ou, ov, ow, oT, occ;
+    ws_field P;          // the block's pressure (null: not kept)
     int n[3];            // interior extents
     int pad;
     double rh[3];        // 1/h per direction
@@ -118,7 +119,9 @@ struct BatchBlock {
 
 // The RK3 stage the last batched RHS launch applies: every block's f is fb and fn fa when
 // `swapped`, else f = fa and fn = fb.  prims: the launch also writes the new primitives to
-// each block's od..occ (and, with planes, re-derives f on the boundary planes).
+// each block's od..oT, with cc the sound speed to occ, with pressure P to the block's P
+// (and, with planes, re-derives f on the boundary planes); cfl_part: each tile leaves the
+// maximum CFL speed of its cells at cfl_part[tile] (its index in the launch's tile list).
 struct BatchStage {
     const double *dt_dev;
     double dt;
@@ -126,6 +129,9 @@ struct BatchStage {
     int swapped;
     int prims = 0;
     int planes = 0;
+    int cc = 1;
+    int pressure = 0;
+    double *cfl_part = nullptr;
 };
 
 // Device tile lists of a batch for one tile shape: entry (block, bx, by, bz) per tile,
@@ -171,8 +177,10 @@ int launch_prepare_batch(const KParams &k, const AuxBlock *tab, int nb, long gho
 int launch_stage_batch(const KParams &k, const AuxBlock *tab, int nb, long cell_wgs, int swapped, bool planes,
                        bool sound, double *cfl_part, hipStream_t s);
 // dts[b] = cfl * hmin / (maximum of block b's CFL partials), then *dt_dev = min over b.
+// Block b's partials are cfl_part[ranges[b].x .. ranges[b].y) when ranges is set (the tiles
+// of a stage-applying launch), else those of its stage-kernel work-groups.
 int launch_cfl_batch(const AuxBlock *tab, int nb, const double *cfl_part, double cfl, double *dts, double *dt_dev,
-                     hipStream_t s);
+                     hipStream_t s, const int2 *ranges = nullptr);
 
 // (tile shape, schedule, direction order) the autotuner found best for these extents and
 // precision; false if none
