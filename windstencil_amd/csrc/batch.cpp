@@ -452,7 +452,6 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
         // d_blocks reads the blocks' fields and writes their second sets, d_blocks2 the reverse
         const BatchBlock *toB = from_second ? b->d_blocks2 : b->d_blocks;
         const BatchBlock *toA = from_second ? b->d_blocks : b->d_blocks2;
-        const int setA = from_second ? 1 : 0, setB = 1 - setA;
         if (prepare && !from_second)
             r = launch_prepare_batch(b->ctx[0]->kp, b->d_aux, nb, b->ghost_wgs, b->cell_wgs, s);
         for (int krk = 1; krk <= 3 && !r; ++krk) {
@@ -469,12 +468,11 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
                     const ws_field own[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
                     r = launch_copy_fields(*b->ctx[i], sec, own, 6, s);
                 }
-            } else if (krk == 3 && cfl_next) {
-                bs.cfl_part = b->d_cfl_tiles;
+            } else {
+                bs.halo = 1;   // the launch fills the output sets' periodic halos itself
+                if (krk == 3 && cfl_next) bs.cfl_part = b->d_cfl_tiles;
             }
             if (!r) r = batch_rhs(b, s, &bs, krk == 2 ? toA : toB);
-            if (!r && !(krk == 3 && last))
-                r = launch_ghost_fill_batch(b->d_aux, nb, b->ghost_wgs, krk == 2 ? setA : setB, s);
         }
         return r;
     }
@@ -489,16 +487,9 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
         if (prims && krk < 3) {
             bs.prims = 1;
             bs.planes = planes;
+            bs.halo = 1;   // the launch fills the written sets' periodic halos itself
+            // stage 1 writes the second sets, stage 2 the state's fields
             if (!r) r = batch_rhs(b, s, &bs, krk == 2 ? b->d_blocks2 : b->d_blocks);
-            // stage 1 wrote the second sets, stage 2 the state's fields
-            if (!r && aux) r = launch_ghost_fill_batch(b->d_aux, nb, b->ghost_wgs, krk == 1 ? 1 : 0, s);
-            for (size_t i = 0; i < b->ctx.size() && !r && !aux; ++i) {
-                const BatchBlock &h = b->host_blocks[i];
-                const ws_state &st = b->st[i];
-                const ws_field g1[6] = {h.od, h.ou, h.ov, h.ow, h.oT, h.occ};
-                const ws_field g2[6] = {st.d, st.u, st.v, st.w, st.T, st.cc};
-                r = launch_ghost_fill(*b->ctx[i], krk == 1 ? g1 : g2, 6, s);
-            }
             continue;
         }
         if (!r) r = batch_rhs(b, s, fuse ? &bs : nullptr);

@@ -354,13 +354,13 @@ int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s,
                 r = launch_copy_fields(*c, second, g, 6, s);
             } else {
                 sa.pcc = out[5];
+                sa.halo = 1;   // the launch fills the output set's periodic halo itself
                 if (krk == 3) sa.cfl_part = cfl_next;
             }
             if (!r)
                 r = c->cfg.precision == WS_PREC_MIXED
                         ? launch_rhs_mixed_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s)
                         : launch_rhs_tiled_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s);
-            if (!r && !(krk == 3 && last)) r = launch_ghost_fill(*c, out, 6, s);
         }
         return r;
     }
@@ -375,13 +375,13 @@ int rk3_step_fused(ws_ctx *c, ws_state *st, const double *dt_dev, hipStream_t s,
             if (krk < 3) {
                 sa.planes = planes;
                 sa.pd = out[0]; sa.pu = out[1]; sa.pv = out[2]; sa.pw = out[3]; sa.pT = out[4]; sa.pcc = out[5];
+                sa.halo = 1;   // the launch fills the output set's periodic halo itself
             }
             if (!r)
                 r = c->cfg.precision == WS_PREC_MIXED
                         ? launch_rhs_mixed_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s)
                         : launch_rhs_tiled_stage(*c, in[0], in[1], in[2], in[3], in[5], st->du, interior_job(c), sa, s);
             if (!r && krk == 3) r = launch_stage(*c, 0, *st, nolap, c->cfg.dt, dt_dev, planes, false, s, cfl_next);
-            if (!r && krk < 3) r = launch_ghost_fill(*c, out, 6, s);
             continue;
         }
         if (fuse) {

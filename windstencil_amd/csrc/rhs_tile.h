@@ -470,6 +470,26 @@ __device__ __forceinline__ double tile_compute(const KParams &k, const ws_metric
                 if (sa.pP.ptr) sa.pP.ptr[fidx(sa.pP.pitch, k.nyg, cx, cy, cz)] = p.e / p.d * (k.gamma - 1.0);
                 if (sa.cfl_part) cfl_max = fmax(cfl_max, fmax(fabs(p.u) + c, fmax(fabs(p.v) + c, fabs(p.w) + c)));
                 const int ix = cx - LAP, iy = cy - LAP, iz = cz - LAP;
+                if (sa.halo) {
+                    // the ghost fill's copies of this cell: ghost g < LAP of a direction holds
+                    // interior n + g, ghost n + LAP + g holds interior LAP + g (ghosted
+                    // coordinates), one direction at a time (faces only); with n < 2 LAP a
+                    // cell feeds both sides
+                    auto put = [&](int gx, int gy, int gz) {
+                        sa.pd.ptr[fidx(sa.pd.pitch, k.nyg, gx, gy, gz)] = p.d;
+                        sa.pu.ptr[fidx(sa.pu.pitch, k.nyg, gx, gy, gz)] = p.u;
+                        sa.pv.ptr[fidx(sa.pv.pitch, k.nyg, gx, gy, gz)] = p.v;
+                        sa.pw.ptr[fidx(sa.pw.pitch, k.nyg, gx, gy, gz)] = p.w;
+                        sa.pT.ptr[fidx(sa.pT.pitch, k.nyg, gx, gy, gz)] = p.T;
+                        if (sa.pcc.ptr) sa.pcc.ptr[fidx(sa.pcc.pitch, k.nyg, gx, gy, gz)] = c;
+                    };
+                    if (ix < LAP) put(cx + k.nx, cy, cz);
+                    if (ix >= k.nx - LAP) put(cx - k.nx, cy, cz);
+                    if (iy < LAP) put(cx, cy + k.ny, cz);
+                    if (iy >= k.ny - LAP) put(cx, cy - k.ny, cz);
+                    if (iz < LAP) put(cx, cy, cz + k.nz);
+                    if (iz >= k.nz - LAP) put(cx, cy, cz - k.nz);
+                }
                 if (sa.planes && (ix == 0 || ix == k.nx - 1 || iy == 0 || iy == k.ny - 1 || iz == 0 || iz == k.nz - 1))
                     plane_conserved(p, k.cv, rv);
             }
@@ -658,6 +678,7 @@ k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int nti
             if (bs.cc) sa.pcc = b.occ;
             if (bs.pressure) sa.pP = b.P;
             sa.cfl_part = bs.cfl_part;
+            sa.halo = bs.halo;
         }
     }
     const double cmax = tile_compute<DIR, L, W, K, false, ACC, MIX, false, STG>(kb, nomf, b.du, g, ld, sp, sa, [] {});

@@ -56,6 +56,10 @@ struct StageArgs {
     int planes = 0;
     ws_field pd{}, pu{}, pv{}, pw{}, pT{}, pcc{}, pP{};
     double *cfl_part = nullptr;
+    // with pd: also write the periodic face halo of pd..pT (and pcc) -- the copies the
+    // ghost fill of those fields would make from the interior cells written here (every
+    // direction periodic)
+    int halo = 0;
 };
 
 
@@ -132,6 +136,7 @@ struct BatchStage {
     int cc = 1;
     int pressure = 0;
     double *cfl_part = nullptr;
+    int halo = 0;        // as StageArgs::halo, for every block
 };
 
 // Device tile lists of a batch for one tile shape: entry (block, bx, by, bz) per tile,
