@@ -416,8 +416,8 @@ int ws_batch_cfl_dt(ws_batch *b, double cfl, double *dt_dev, hipStream_t s)
 // cfl_stage_buffer; the alternating modes: its stage-3 launch, per tile in d_cfl_tiles).
 // mode and prepare as for a single box's rk3_step_fused (stage fusion 3): an alternating
 // step from set A runs stage 1 into B, stage 2 into A and stage 3 into B with the batched
-// z launch writing every block's primitives and sound speed, one batched ghost fill after
-// each stage; the last step of a run from the second sets copies them to the blocks'
+// z launch writing every block's primitives, sound speed and their periodic face halo;
+// the last step of a run from the second sets copies them to the blocks'
 // fields and lets the stage-3 launch write the primitives and P over their interior.
 static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl_next, StepMode mode = STEP_PLAIN,
                       bool prepare = true)
@@ -437,7 +437,7 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
         fuse = b->st[i].f.ptr == (swapped ? b->fbs[i] : b->fas[i]);
     // Fusion level 2 (as for a single box): the z launch of stages 1 and 2 also writes the
     // primitives, stage 1 from the blocks' own fields into their second sets (d_blocks),
-    // stage 2 back (d_blocks2); a ghost fill of the six written fields follows.
+    // stage 2 back (d_blocks2), with the periodic face halo of the six written fields.
     const bool prims = fuse && b->ctx[0]->fuse_stage > 1 && prim_tables(b, s);
     // the helper kernels of all blocks in one launch each (the table holds the second
     // primitive sets when prims is on: prim_tables runs first and aux_table follows it)
@@ -477,7 +477,8 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
         return r;
     }
     // the blocks are periodic (ws_batch_create); stages 1 and 2 leave the next stage's
-    // inputs prepared (fused stage kernel writing the sound speed, then one ghost fill)
+    // inputs prepared (the z launch or the stage kernel writing the sound speed, the halo by
+    // the z launch itself or a ghost fill after the stage kernel)
     for (int krk = 1; krk <= 3 && !r; ++krk) {
         if (krk == 1) {
             if (aux) r = launch_prepare_batch(b->ctx[0]->kp, b->d_aux, nb, b->ghost_wgs, b->cell_wgs, s);

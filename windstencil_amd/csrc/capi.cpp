@@ -248,8 +248,8 @@ int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s)
 // to the context's second primitive set: stage 1 reads the state's fields and writes the
 // second set, stage 2 reads the second set and writes the state's d, u, v, w, T and cc,
 // and stage 3 ends with the stage kernel (primitives, P and the boundary planes of the
-// state).  A ghost fill of the six fields follows stages 1 and 2, so the state ends the
-// step with the bits of the unfused step.  The second set is allocated by the first eager
+// state).  That launch also writes the periodic face halo of the six fields it fills (the
+// ghost fill's copies), so the state ends the step with the bits of the unfused step.  The second set is allocated by the first eager
 // step; a step captured before that runs without it.
 int prim_set(ws_ctx *c, hipStream_t s, ws_field out[6])
 {
@@ -320,8 +320,8 @@ bool alternating_steps(ws_ctx *c, hipStream_t s, bool cfl)
 // Stage fusion 3 (modes other than STEP_PLAIN): a step from set A (the state's fields or
 // the second set, ghost-filled, sound speed included) runs stage 1 into B, stage 2 into A
 // and stage 3 into B, each stage's last RHS launch writing the primitives and sound speed of
-// its new f (and in stage 3 the CFL maxima of each tile), and one ghost fill of six fields
-// follows each stage: the next step starts from B with nothing to prepare, and the
+// its new f and their periodic face halo (and in stage 3 the CFL maxima of each tile): the
+// next step starts from B with nothing to prepare, and the
 // stage kernel after stage 3 and the next step's preparation are gone.  The last step of a
 // run from the second set copies that set (the stage-2 primitives, ghost layers and sound
 // speed) to the state's fields and lets the stage-3 launch write the primitives and P of
