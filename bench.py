@@ -54,6 +54,9 @@ def parse():
     ap.add_argument("--tile", type=int, default=-1, help="force a tile shape (default: autotune)")
     ap.add_argument("--schedule", type=int, default=-1,
                     help="with --tile: force a schedule (0 grid, 1 persistent, 2 persistent + prefetch)")
+    ap.add_argument("--order", type=int, default=-1,
+                    help="rk3: force the stage order after tuning (0 x, y, z: z applies the stage; "
+                         "1 z, y, x: x applies it)")
     ap.add_argument("--workload", default="rhs", choices=["rhs", "batched", "rk3"],
                     help="rhs: the headline RHS line (default); batched: 48 uneven blocks; "
                          "rk3: SSP-RK3 steps of one box in a graph")
@@ -315,14 +318,15 @@ def bench_batched(args):
 
 
 def _ctx_choice(ctx):
-    """'shape/schedule/order' the context runs its tiled RHS with."""
+    """'shape/schedule/order/stage order' the context runs its tiled RHS with."""
     import windstencil_amd as W
     L = W.lib()
-    chosen, sched, order = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    chosen, sched, order, sorder = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
     W.check(L.ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
     W.check(L.ws_ctx_get_direction_order(ctx.handle, ctypes.byref(order)))
-    return f"{chosen.value}/{sched.value}/{order.value}"
+    W.check(L.ws_ctx_get_stage_order(ctx.handle, ctypes.byref(sorder)))
+    return f"{chosen.value}/{sched.value}/{order.value}/{sorder.value}"
 
 
 def bench_rk3(args):
@@ -356,6 +360,8 @@ def bench_rk3(args):
     else:
         W.check(L.ws_du_comput(ctx.handle, ctypes.byref(st.ws), None, sh))
         W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 3, None, sh))
+    if args.order >= 0:
+        W.check(L.ws_ctx_set_stage_order(ctx.handle, args.order))
     W.check(L.ws_rk3_run(ctx.handle, ctypes.byref(st.ws), 2, 0.5, 1, sh))     # warm-up steps
     st.sync_from_ws()
     torch.cuda.synchronize()
@@ -380,7 +386,7 @@ def bench_rk3(args):
                    "cells": N},
                   roofline=roof, rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
                   mcell_steps_per_s=round(N * nsteps / secs / 1e6, 2),
-                  stage_fusion=args.stage_fusion, **{"tile_shape/schedule/order": _ctx_choice(ctx)},
+                  stage_fusion=args.stage_fusion, **{"tile_shape/schedule/order/stage_order": _ctx_choice(ctx)},
                   totals=[float(v) for v in f.sum(dim=(1, 2, 3)).cpu()])
     out["steps"] = nsteps
     print(json.dumps(out))

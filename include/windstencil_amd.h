@@ -189,6 +189,13 @@ int ws_ctx_get_schedule(const ws_ctx *ctx, int32_t *schedule);
  * precision (the autotuner picks) */
 int ws_ctx_set_direction_order(ws_ctx *ctx, int32_t order);
 int ws_ctx_get_direction_order(const ws_ctx *ctx, int32_t *order);
+/* WS_ORDER_* of the RHS launches whose last one applies an RK3 stage (stage fusion): that
+ * launch also carries the stage's reads and writes, so the faster order can differ from
+ * the plain RHS's (the autotuner picks; the default is WS_ORDER_XYZ).  Stage-applying
+ * steps equal the unfused ones bit for bit when this order equals the direction order,
+ * to rounding otherwise.  No counterpart in the reference. */
+int ws_ctx_set_stage_order(ws_ctx *ctx, int32_t order);
+int ws_ctx_get_stage_order(const ws_ctx *ctx, int32_t *order);
 /* 2: RK3 steps let the last directional RHS launch apply each stage's update (tiled
  * kernels, grid schedule, constant metrics, no closures, no viscous terms; same result as
  * the separate stage kernel, du then holds partial sums) and, with every direction
@@ -277,9 +284,13 @@ int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipSt
 /* Time every (tile shape, schedule, direction order) of the tiled kernels, in the context's
  * precision, on this state (two untimed and `reps` timed RHS launches each, HIP events on
  * `stream`), keep the fastest for the context and remember it for contexts of the same
- * extents and precision.  times_ms (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major,
- * each the faster of the two orders) may be NULL.  Overwrites st->du; synchronises the
- * stream.  No counterpart in the reference. */
+ * extents and precision.  When RK3 steps of that choice apply their stages in the last RHS
+ * launch (ws_ctx_set_stage_fusion), the stage order is then timed the same way: the RHS of
+ * an RK3 stage 2 whose last launch writes the new f, primitives, sound speed and halo into
+ * scratch the call allocates and frees (the state's f is read, not written).  times_ms
+ * (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major, each the faster of the two orders)
+ * may be NULL.  Overwrites st->du; synchronises the stream.  No counterpart in the
+ * reference. */
 int ws_autotune(ws_ctx *ctx, const ws_state *st, int32_t reps, float *times_ms, hipStream_t stream);
 
 /* Vector ALU throughput of the current device in tera lane-operations per second: eight

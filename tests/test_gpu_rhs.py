@@ -70,7 +70,8 @@ def test_smallest_and_ragged_extents(orc, ws, shape):
 
 def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
     """All tile shapes give the same RHS to rounding; the autotuned context still
-    matches the oracle and remembers its choice for the grid."""
+    matches the oracle and remembers its choice (stage order included) for the grid; the
+    tuning leaves the state's f and primitives as they were."""
     import ctypes
     nx, ny, nz = 90, 64, 32
     prims = random_state(nx, ny, nz, 31, "shock")
@@ -85,22 +86,32 @@ def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
     ctx = make_ctx(nx, ny, nz)
     st = upload_state(prims)
     gpu_du_comput(ctx, st)
+    W.check(W.lib().ws_pri_to_cons(ctx.handle, st.ws.f, st.ws.d, st.ws.u, st.ws.v, st.ws.w, st.ws.T,
+                                   W.job((0, 0, 0), (nx, ny, nz)), W.stream_handle()))
+    kept = ("f", "d", "u", "v", "w", "T", "cc")
+    before = {k: st.t[k].clone() for k in kept}
     ns = len(W.SCHEDULES)
     times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
     W.check(W.lib().ws_autotune(ctx.handle, ctypes.byref(st.ws), 3, times, W.stream_handle()))
+    for k in kept:   # the stage-order timing writes scratch only
+        assert st.t[k].equal(before[k]), k
     log(f"  autotune ms {list(times)}")
-    chosen, sched, order = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    chosen, sched, order, sorder = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
     W.check(W.lib().ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
     W.check(W.lib().ws_ctx_get_direction_order(ctx.handle, ctypes.byref(order)))
+    W.check(W.lib().ws_ctx_get_stage_order(ctx.handle, ctypes.byref(sorder)))
+    log(f"  chose shape {chosen.value} schedule {sched.value} order {order.value} stage order {sorder.value}")
     assert times[chosen.value * ns + sched.value] == min(times)
-    assert order.value in W.ORDERS
+    assert order.value in W.ORDERS and sorder.value in W.ORDERS
     ctx2 = make_ctx(nx, ny, nz)
-    again, again_s, again_o = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    again, again_s, again_o, again_so = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_ctx_get_tile_shape(ctx2.handle, ctypes.byref(again)))
     W.check(W.lib().ws_ctx_get_schedule(ctx2.handle, ctypes.byref(again_s)))
     W.check(W.lib().ws_ctx_get_direction_order(ctx2.handle, ctypes.byref(again_o)))
-    assert (again.value, again_s.value, again_o.value) == (chosen.value, sched.value, order.value)
+    W.check(W.lib().ws_ctx_get_stage_order(ctx2.handle, ctypes.byref(again_so)))
+    assert (again.value, again_s.value, again_o.value, again_so.value) == \
+        (chosen.value, sched.value, order.value, sorder.value)
     assert rel_l2(gpu_du_comput(ctx2, upload_state(prims)), ref) <= TOL_FP64
 
 

@@ -95,6 +95,8 @@ _SIGS = {
     "ws_ctx_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_ctx_set_direction_order": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_ctx_get_direction_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_ctx_set_stage_order": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_ctx_get_stage_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_ctx_set_stage_fusion": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_state_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_state_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
@@ -266,9 +268,14 @@ class Context:
     def set_direction_order(self, o):
         check(lib().ws_ctx_set_direction_order(self.handle, o), "ws_ctx_set_direction_order")
 
+    def set_stage_order(self, o):
+        """Direction order of the RHS launches that apply an RK3 stage (WS_ORDER_*)."""
+        check(lib().ws_ctx_set_stage_order(self.handle, o), "ws_ctx_set_stage_order")
+
     def set_stage_fusion(self, level):
-        """0: separate stage kernel; 1: the last RHS launch applies each stage; 2 (default):
-        it also derives the primitives of stages 1 and 2."""
+        """0: separate stage kernel; 1: the last RHS launch applies each stage; 2: it also
+        derives the primitives of stages 1 and 2; 3 (default): within ws_rk3_run those of
+        stage 3 too, the steps alternating two primitive sets."""
         check(lib().ws_ctx_set_stage_fusion(self.handle, int(level)), "ws_ctx_set_stage_fusion")
 
 

@@ -2,9 +2,11 @@
 //
 // Every (tile shape, schedule, direction order) runs the full inviscid RHS on the caller's
 // state, in the context's precision: two untimed launches, then `reps` launches bracketed
-// by HIP events on the caller's stream.  The fastest becomes the context's and is
-// remembered for the grid extents and precision, so later contexts of the same size and
-// precision start from it.
+// by HIP events on the caller's stream.  The fastest becomes the context's.  When RK3
+// steps of that choice let the last RHS launch apply the stage, both direction orders of
+// that stage-applying RHS are timed the same way (into scratch outputs) and the faster
+// becomes the stage order.  The choice is remembered for the grid extents and precision,
+// so later contexts of the same size and precision start from it.
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -20,7 +22,7 @@ namespace {
 
 std::mutex g_mu;
 struct Choice {
-    int shape, schedule, order;
+    int shape, schedule, order, stage_order;
 };
 // (nx, ny, nz, precision) -> choice
 std::map<std::tuple<int, int, int, int>, Choice> g_best;
@@ -32,11 +34,96 @@ int launch(const Ctx &c, const ws_state *st, const ws_job &job, hipStream_t s)
     return launch_rhs_tiled(c, st->d, st->u, st->v, st->w, st->cc, nullptr, st->du, job, 0, s);
 }
 
+// Mean time of `reps` calls of fn() on s after two untimed ones; negative on error (r set).
+template <class F>
+float time_launches(F fn, int reps, hipEvent_t e0, hipEvent_t e1, hipStream_t s, int &r)
+{
+    for (int i = 0; i < 2 && !r; ++i) r = fn();
+    if (r) return -1.f;
+    hipEventRecord(e0, s);
+    for (int i = 0; i < reps && !r; ++i) r = fn();
+    hipEventRecord(e1, s);
+    if (r) return -1.f;
+    hipError_t e = hipEventSynchronize(e1);
+    if (e != hipSuccess) {
+        r = hip_error(e, "autotune sync");
+        return -1.f;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    return ms / (float)reps;
+}
+
+// The stage order: the RHS of an RK3 stage 2 (f = 3/4 fn + 1/4 (f + dt RHS)) whose last
+// launch writes the new f and, with every direction periodic, the primitives, sound speed,
+// their periodic halo and the per-tile CFL maxima: what a fused step's stage launches
+// write.  fn is the state's f (read only); f, the primitives and the maxima go to scratch
+// allocated here (f a copy of the state's, so the values stay physical).  Returns the
+// faster order, or -1 when the scratch cannot be had (the order then stays as it was).
+int tune_stage_order(ws_ctx *c, const ws_state *st, const ws_job &job, int reps, hipEvent_t e0, hipEvent_t e1,
+                     hipStream_t s, int &r)
+{
+    const bool periodic = c->cfg.periodic[0] && c->cfg.periodic[1] && c->cfg.periodic[2];
+    const bool prims = periodic && c->fuse_stage > 1;
+    const size_t fper = (size_t)st->f.pitch * c->kp.ny * c->kp.nz;
+    const size_t pper = (size_t)st->d.pitch * c->kp.nyg * c->kp.nzg;
+    const int saved = c->stage_order;
+    long tiles = 0;
+    for (int o = 0; o < WS_ORDER_COUNT; ++o) {
+        c->stage_order = o;
+        const long n = stage_launch_tiles(*c);
+        if (n > tiles) tiles = n;
+    }
+    c->stage_order = saved;
+    const size_t elems = 5 * fper + (prims ? 6 * pper + (size_t)tiles : 0);
+    double *scratch = nullptr;
+    if (hipMalloc(&scratch, elems * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    hipError_t e = hipMemcpyAsync(scratch, st->f.ptr, 5 * fper * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+        r = hip_error(e, "autotune stage scratch copy");
+        hipFree(scratch);
+        return -1;
+    }
+    ws_soa f = st->f;
+    f.ptr = scratch;
+    StageArgs sa{f, st->f, nullptr, c->cfg.dt, 2};
+    sa.planes = c->cfg.bc_planes != 0;
+    if (prims) {
+        ws_field *out[6] = {&sa.pd, &sa.pu, &sa.pv, &sa.pw, &sa.pT, &sa.pcc};
+        for (int q = 0; q < 6; ++q) *out[q] = ws_field{scratch + 5 * fper + q * pper, st->d.pitch, 0};
+        sa.halo = 1;
+        sa.cfl_part = scratch + 5 * fper + 6 * pper;
+    }
+    auto stage = [&]() {
+        return c->cfg.precision == WS_PREC_MIXED
+                   ? launch_rhs_mixed_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, job, sa, s)
+                   : launch_rhs_tiled_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, job, sa, s);
+    };
+    int best = -1;
+    float best_ms = 1e30f;
+    for (int o = 0; o < WS_ORDER_COUNT && !r; ++o) {
+        c->stage_order = o;
+        const float ms = time_launches(stage, reps, e0, e1, s, r);
+        if (!r && ms < best_ms) {
+            best_ms = ms;
+            best = o;
+        }
+    }
+    c->stage_order = saved;
+    hipStreamSynchronize(s);
+    hipFree(scratch);
+    return r ? -1 : best;
+}
+
 }  // namespace
 
 namespace ws {
 
-bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order)
+bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order,
+                       int *stage_order)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_best.find(std::make_tuple(nx, ny, nz, precision));
@@ -44,6 +131,7 @@ bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *s
     *shape = it->second.shape;
     *schedule = it->second.schedule;
     *order = it->second.order;
+    *stage_order = it->second.stage_order;
     return true;
 }
 
@@ -60,10 +148,11 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
     if (e != hipSuccess) return hip_error(e, "autotune event");
     e = hipEventCreate(&e1);
     if (e != hipSuccess) { hipEventDestroy(e0); return hip_error(e, "autotune event"); }
-    const Choice saved{c->tile_shape, c->schedule, c->dir_order};
+    const Choice saved{c->tile_shape, c->schedule, c->dir_order, c->stage_order};
     Choice best = saved;
     int r = WS_OK;
     float best_ms = 1e30f;
+    auto rhs = [&]() { return launch(*c, st, job, s); };
     for (int shape = 0; shape < WS_TILE_COUNT && !r; ++shape)
         for (int sched = 0; sched < WS_SCHED_COUNT && !r; ++sched) {
             float pair_ms = 1e30f;
@@ -71,28 +160,29 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
                 c->tile_shape = shape;
                 c->schedule = sched;
                 c->dir_order = order;
-                for (int i = 0; i < 2 && !r; ++i) r = launch(*c, st, job, s);
+                const float ms = time_launches(rhs, reps, e0, e1, s, r);
                 if (r) break;
-                hipEventRecord(e0, s);
-                for (int i = 0; i < reps && !r; ++i) r = launch(*c, st, job, s);
-                hipEventRecord(e1, s);
-                if (r) break;
-                e = hipEventSynchronize(e1);
-                if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, e0, e1);
-                ms /= (float)reps;
                 if (ms < pair_ms) pair_ms = ms;
-                if (ms < best_ms) { best_ms = ms; best = Choice{shape, sched, order}; }
+                if (ms < best_ms) { best_ms = ms; best = Choice{shape, sched, order, saved.stage_order}; }
             }
             if (times_ms) times_ms[shape * WS_SCHED_COUNT + sched] = pair_ms;
         }
+    if (!r) {
+        c->tile_shape = best.shape;
+        c->schedule = best.schedule;
+        c->dir_order = best.order;
+        if (fused_stage_ok(c)) {
+            const int o = tune_stage_order(c, st, job, reps, e0, e1, s, r);
+            if (o >= 0) best.stage_order = o;
+        }
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     const Choice &use = r ? saved : best;
     c->tile_shape = use.shape;
     c->schedule = use.schedule;
     c->dir_order = use.order;
+    c->stage_order = use.stage_order;
     if (!r) {
         std::lock_guard<std::mutex> lk(g_mu);
         g_best[std::make_tuple(c->cfg.nx, c->cfg.ny, c->cfg.nz, c->cfg.precision)] = best;

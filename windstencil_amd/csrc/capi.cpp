@@ -453,11 +453,12 @@ int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
     if (!c) return set_error(WS_ERR_NOMEM, "host allocation failed");
     c->cfg = *cfg;
     c->kp = make_kparams(*cfg);
-    int shape, sched, order;
-    if (remembered_choice(cfg->nx, cfg->ny, cfg->nz, cfg->precision, &shape, &sched, &order)) {
+    int shape, sched, order, stage_order;
+    if (remembered_choice(cfg->nx, cfg->ny, cfg->nz, cfg->precision, &shape, &sched, &order, &stage_order)) {
         c->tile_shape = shape;
         c->schedule = sched;
         c->dir_order = order;
+        c->stage_order = stage_order;
     }
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return hip_error(e, "hipGetDevice"); }
@@ -498,11 +499,12 @@ int ws_ctx_set_precision(ws_ctx *c, int32_t precision)
     if (!c) return set_error(WS_ERR_INVALID, "null context");
     if (precision != WS_PREC_FP64 && precision != WS_PREC_MIXED) return set_error(WS_ERR_INVALID, "unknown precision");
     c->cfg.precision = precision;
-    int shape, sched, order;
-    if (remembered_choice(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision, &shape, &sched, &order)) {
+    int shape, sched, order, stage_order;
+    if (remembered_choice(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision, &shape, &sched, &order, &stage_order)) {
         c->tile_shape = shape;
         c->schedule = sched;
         c->dir_order = order;
+        c->stage_order = stage_order;
     }
     return WS_OK;
 }
@@ -550,6 +552,21 @@ int ws_ctx_get_direction_order(const ws_ctx *c, int32_t *order)
 {
     if (!c || !order) return set_error(WS_ERR_INVALID, "null argument");
     *order = c->dir_order;
+    return WS_OK;
+}
+
+int ws_ctx_set_stage_order(ws_ctx *c, int32_t order)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (order < 0 || order >= WS_ORDER_COUNT) return set_error(WS_ERR_INVALID, "unknown direction order");
+    c->stage_order = order;
+    return WS_OK;
+}
+
+int ws_ctx_get_stage_order(const ws_ctx *c, int32_t *order)
+{
+    if (!c || !order) return set_error(WS_ERR_INVALID, "null argument");
+    *order = c->stage_order;
     return WS_OK;
 }
 

@@ -22,11 +22,11 @@ long stage_launch_tiles(const Ctx &c)
     for (int d = 0; d < 3; ++d) box.s[d] = LAP;
     box.e[0] = c.kp.nx + LAP; box.e[1] = c.kp.ny + LAP; box.e[2] = c.kp.nz + LAP;
     switch (c.tile_shape) {
-    case 0: return stage_tiles_n<0>(box, c.dir_order);
-    case 1: return stage_tiles_n<1>(box, c.dir_order);
-    case 2: return stage_tiles_n<2>(box, c.dir_order);
-    case 3: return stage_tiles_n<3>(box, c.dir_order);
-    default: return stage_tiles_n<4>(box, c.dir_order);
+    case 0: return stage_tiles_n<0>(box, c.stage_order);
+    case 1: return stage_tiles_n<1>(box, c.stage_order);
+    case 2: return stage_tiles_n<2>(box, c.stage_order);
+    case 3: return stage_tiles_n<3>(box, c.stage_order);
+    default: return stage_tiles_n<4>(box, c.stage_order);
     }
 }
 

@@ -895,13 +895,13 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
     box.e[0] = (int)job.end.x; box.e[1] = (int)job.end.y; box.e[2] = (int)job.end.z;
     ws_metric_fields mf{};
     const int shape = c.tile_shape;
+    const int order = sa ? c.stage_order : c.dir_order;
     if (m) {
         mf = *m;
-        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order,
-                                nullptr, s);
+        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, order, nullptr,
+                                s);
     } else {
-        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, c.dir_order, sa,
-                                 s);
+        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, order, sa, s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, MIX ? "mixed-precision RHS launch" : "tiled RHS launch");

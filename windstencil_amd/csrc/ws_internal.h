@@ -77,8 +77,11 @@ struct Ctx {
     int tile_shape = WS_TILE_30x16_K2;
     // how the tiled kernels walk their tiles (WS_SCHED_*)
     int schedule = WS_SCHED_GRID;
-    // order of the directional launches (WS_ORDER_*)
+    // order of the directional launches (WS_ORDER_*); stage_order: that of the launches
+    // that apply an RK3 stage (the stage kernel's share moves into the last of them, so
+    // its best order can differ from the plain RHS's)
     int dir_order = WS_ORDER_XYZ;
+    int stage_order = WS_ORDER_XYZ;
     // RK3 steps apply each stage's update in the last RHS launch where they can (1), there
     // also derive the primitives of stages 1 and 2 (2), and within ws_rk3_run those of
     // stage 3 too, alternating the primitive sets from step to step (3)
@@ -187,9 +190,10 @@ int launch_stage_batch(const KParams &k, const AuxBlock *tab, int nb, long cell_
 int launch_cfl_batch(const AuxBlock *tab, int nb, const double *cfl_part, double cfl, double *dts, double *dt_dev,
                      hipStream_t s, const int2 *ranges = nullptr);
 
-// (tile shape, schedule, direction order) the autotuner found best for these extents and
-// precision; false if none
-bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order);
+// (tile shape, schedule, direction order, stage order) the autotuner found best for these
+// extents and precision; false if none
+bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order,
+                       int *stage_order);
 
 // error reporting (thread-local message)
 int set_error(int code, const std::string &msg);
@@ -225,6 +229,9 @@ int rhs_for_ctx(ws_ctx *c, const ws_state *st, hipStream_t s);
 // du_comput on inputs already prepared (ghosted primitives and sound speed): the inviscid
 // RHS, plus the viscous terms when the configuration has them
 int du_from_prepared(ws_ctx *c, const ws_state *st, hipStream_t s);
+// whether RK3 steps can let the last RHS launch apply the stage (stage fusion on, tiled
+// kernels, grid schedule, no closures, no viscous terms)
+bool fused_stage_ok(const ws_ctx *c);
 // How a step of stage fusion 3 treats the primitive sets (rk3_step_fused): STEP_PLAIN starts
 // and ends with the primitives in the state's fields (every level); STEP_FROM_STATE and
 // STEP_FROM_SECOND start from the state's fields or the context's second set, prepared, and
