@@ -55,8 +55,8 @@ def parse():
     ap.add_argument("--schedule", type=int, default=-1,
                     help="with --tile: force a schedule (0 grid, 1 persistent, 2 persistent + prefetch)")
     ap.add_argument("--order", type=int, default=-1,
-                    help="rk3: force the stage order after tuning (0 x, y, z: z applies the stage; "
-                         "1 z, y, x: x applies it)")
+                    help="rk3 and batched: force the stage order after tuning (0 x, y, z: z applies "
+                         "the stage; 1 z, y, x: x applies it)")
     ap.add_argument("--workload", default="rhs", choices=["rhs", "batched", "rk3"],
                     help="rhs: the headline RHS line (default); batched: 48 uneven blocks; "
                          "rk3: SSP-RK3 steps of one box in a graph")
@@ -276,6 +276,10 @@ def bench_batched(args):
     W.check(L.ws_batch_get_tile_shape(b.handle, ctypes.byref(chosen)))
     sched = ctypes.c_int32()
     W.check(L.ws_batch_get_schedule(b.handle, ctypes.byref(sched)))
+    if args.order >= 0:
+        W.check(L.ws_batch_set_stage_order(b.handle, args.order))
+    sorder = ctypes.c_int32()
+    W.check(L.ws_batch_get_stage_order(b.handle, ctypes.byref(sorder)))
     for _ in range(args.warmup):
         W.check(L.ws_batch_du_comput(b.handle, sh))
     torch.cuda.synchronize()
@@ -310,7 +314,7 @@ def bench_batched(args):
                             "kernel_ms": round(kern_ms, 4)},
                   tile_shape=chosen.value, schedule=sched.value, autotune_ms=tune,
                   autotune_key="tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
-                  rk3={"steps": nsteps, "graph": True, "seconds": round(rk3_s, 4),
+                  rk3={"steps": nsteps, "graph": True, "seconds": round(rk3_s, 4), "stage_order": sorder.value,
                        "mcell_updates_per_s": round(N * 3 * nsteps / rk3_s / 1e6, 2),
                        "mcell_steps_per_s": round(N * nsteps / rk3_s / 1e6, 2),
                        "note": "N x 3 stages x steps / wall time; CFL 0.5 from the per-block minimum each step"})

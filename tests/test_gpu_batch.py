@@ -104,15 +104,19 @@ def _to_cons(ctx, st):
                              W.job((0, 0, 0), (nx, ny, nz)), W.stream_handle()))
 
 
-@pytest.mark.parametrize("schedule", [W.WS_SCHED_GRID, W.WS_SCHED_PREFETCH])
+@pytest.mark.parametrize("schedule,stage_order", [(W.WS_SCHED_GRID, W.WS_ORDER_XYZ),
+                                                  (W.WS_SCHED_GRID, W.WS_ORDER_ZYX),
+                                                  (W.WS_SCHED_PREFETCH, W.WS_ORDER_XYZ)])
 @pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
-def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision, schedule):
-    """With the grid schedule the batched z launch applies each RK3 stage itself; with the
-    prefetching schedule the stage kernels do: both match single-box runs."""
+def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision, schedule, stage_order):
+    """With the grid schedule the batched z launch (stage order XYZ) or x launch (ZYX)
+    applies each RK3 stage itself; with the prefetching schedule the stage kernels do: all
+    match single-box runs of the same stage order."""
     extents = _extents(13, 5, 6, 33)
     dt = 2e-4
     b, prims = _batch(extents, 500, precision=precision, dt=dt)
     W.check(W.lib().ws_batch_set_schedule(b.handle, schedule))
+    W.check(W.lib().ws_batch_set_stage_order(b.handle, stage_order))
     for e, st in zip(extents, b.states):
         _to_cons(make_ctx(*e), st)
     log(f"batch rk3 fixed dt {extents}")
@@ -120,6 +124,7 @@ def test_batch_rk3_fixed_dt_equals_single_block_runs(ws, precision, schedule):
     b.sync_states()
     for e, p, st in zip(extents, prims, b.states):
         ctx = make_ctx(*e, precision=precision, dt=dt)
+        ctx.set_stage_order(stage_order)
         ref = upload_state(p)
         _to_cons(ctx, ref)
         W.check(W.lib().ws_rk3_run(ctx.handle, ctypes.byref(ref.ws), 6, -1.0, 0, W.stream_handle()))
@@ -173,23 +178,26 @@ def test_batch_stage_fusion_levels(ws, precision, graph, cfl):
         assert err <= (1e-14 if precision == W.WS_PREC_FP64 else 1e-9), (i, err)
 
 
+@pytest.mark.parametrize("stage_order", [W.WS_ORDER_XYZ, W.WS_ORDER_ZYX])
 @pytest.mark.parametrize("nsteps", [2, 3, 4])
 @pytest.mark.parametrize("cfl", [-1.0, 0.5])
 @pytest.mark.parametrize("graph", [0, 1])
-def test_batch_alternating_primitive_sets_equal_level_2(ws, graph, cfl, nsteps):
-    """Stage fusion 3 in a batch: the z launch of stage 3 also writes every block's
-    primitives and sound speed, with the CFL maxima per tile (reduced per block over the
-    block's range of the tile list), and the primitive sets alternate from step to step; a
-    run's last step ends in the blocks' fields (from the second sets: after copying them
-    there).  The same kernels as level 2, so f, every ghosted primitive field and P agree
-    bit for bit with level 2 after 2, 3 and 4 steps (both last-step forms), eager and
-    graph, with a fixed dt and with the CFL step."""
+def test_batch_alternating_primitive_sets_equal_level_2(ws, graph, cfl, nsteps, stage_order):
+    """Stage fusion 3 in a batch: the stage-applying launch of stage 3 (z for stage order
+    XYZ, x for ZYX) also writes every block's primitives and sound speed, with the CFL
+    maxima per tile (reduced per block over the block's range of that launch's tile list),
+    and the primitive sets alternate from step to step; a run's last step ends in the
+    blocks' fields (from the second sets: after copying them there).  The same kernels as
+    level 2, so f, every ghosted primitive field and P agree bit for bit with level 2 after
+    2, 3 and 4 steps (both last-step forms), eager and graph, with a fixed dt and with the
+    CFL step."""
     extents = _extents(29, 5, 6, 30)
     kinds = [k for k in KINDS if k != "rough"]
     runs = []
     for level in (2, 3):
         b, _ = _batch(extents, 710, kinds=kinds, bc_planes=1, dt=2e-4)
         W.check(W.lib().ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
+        W.check(W.lib().ws_batch_set_stage_order(b.handle, stage_order))
         W.check(W.lib().ws_batch_set_stage_fusion(b.handle, level))
         for e, st in zip(extents, b.states):
             _to_cons(make_ctx(*e), st)
@@ -258,20 +266,33 @@ def test_batch_graph_replay_equals_eager(ws, nsteps, schedule):
 
 
 def test_batch_autotune_keeps_results(ws):
+    """The tuned batch gives the same RHS to rounding; tuning (the stage order's timing
+    included, when the grid schedule wins) leaves every block's f and primitives as they
+    were."""
     extents = _extents(19, 5, 20, 60)
     b, _ = _batch(extents, 700)
     s = W.stream_handle()
+    for e, st in zip(extents, b.states):
+        _to_cons(make_ctx(*e), st)
     W.check(W.lib().ws_batch_du_comput(b.handle, s))
     _sync()
     before = [_du(st) for st in b.states]
+    kept = ("f", "d", "u", "v", "w", "T", "cc")
+    state = [{k: st.t[k].clone() for k in kept} for st in b.states]
     ns = len(W.SCHEDULES)
     times = (ctypes.c_float * (len(W.TILE_SHAPES) * ns))()
     W.check(W.lib().ws_batch_autotune(b.handle, 2, times, s))
-    shape, sched = ctypes.c_int32(), ctypes.c_int32()
+    _sync()
+    for i, st in enumerate(b.states):
+        for k in kept:
+            assert st.t[k].equal(state[i][k]), (i, k)
+    shape, sched, sorder = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_batch_get_tile_shape(b.handle, ctypes.byref(shape)))
     W.check(W.lib().ws_batch_get_schedule(b.handle, ctypes.byref(sched)))
-    log(f"  batch autotune ms {list(times)} -> {shape.value}/{sched.value}")
+    W.check(W.lib().ws_batch_get_stage_order(b.handle, ctypes.byref(sorder)))
+    log(f"  batch autotune ms {list(times)} -> {shape.value}/{sched.value}, stage order {sorder.value}")
     assert times[shape.value * ns + sched.value] == min(times)
+    assert sorder.value in W.ORDERS
     W.check(W.lib().ws_batch_du_comput(b.handle, s))
     _sync()
     for a, st in zip(before, b.states):

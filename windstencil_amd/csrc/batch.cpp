@@ -29,6 +29,9 @@ struct ws_batch {
     int kernel = WS_KERNEL_TILED;
     int tile_shape = WS_TILE_30x16_K2;
     int schedule = WS_SCHED_PREFETCH;
+    // which launch of a stage-applying RHS applies the stage (WS_ORDER_*: the last
+    // direction; ws_batch_autotune picks)
+    int stage_order = WS_ORDER_XYZ;
     std::vector<ws_ctx *> ctx;
     std::vector<ws_state> st;
     BatchBlock *d_blocks = nullptr;
@@ -49,9 +52,10 @@ struct ws_batch {
     long ghost_wgs = 0, cell_wgs = 0;
     double *d_cfl_part = nullptr;   // the CFL maxima of the batched stage kernel (cell_wgs)
     bool cfl_in_batch = false;      // the last step's stage kernel wrote them there
-    // per tile shape: each block's range of the z tile list (the stage-applying launch's)
-    int2 *d_tile_ranges[WS_TILE_COUNT] = {};
-    // the CFL maxima per tile of stage fusion 3's stage-3 launch (the longest z list)
+    // per tile shape and stage order: each block's range of the tile list of the
+    // stage-applying launch (z's for WS_ORDER_XYZ, x's for WS_ORDER_ZYX)
+    int2 *d_tile_ranges[WS_TILE_COUNT][WS_ORDER_COUNT] = {};
+    // the CFL maxima per tile of stage fusion 3's stage-3 launch (the longest x or z list)
     double *d_cfl_tiles = nullptr;
     hipStream_t capture_stream = nullptr;
     long long cells = 0;
@@ -89,6 +93,7 @@ BatchTiles tiles_of(const ws_batch *b)
     t.schedule = b->schedule;
     t.blocks = b->d_blocks;
     t.stage = nullptr;
+    t.stage_order = b->stage_order;
     for (int d = 0; d < 3; ++d) {
         t.tiles[d] = b->d_tiles[b->tile_shape][d];
         t.ntiles[d] = b->ntiles[b->tile_shape][d];
@@ -269,10 +274,11 @@ int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *exten
                 block_tiles(i, host[i].n, shape, dir, list);
                 ranges[i].y = (int)list.size();
             }
-            if (dir == 2) {
-                e = hipMalloc(&b->d_tile_ranges[shape], sizeof(int2) * nblocks);
+            if (dir != 1) {
+                int2 *&dr = b->d_tile_ranges[shape][dir == 2 ? WS_ORDER_XYZ : WS_ORDER_ZYX];
+                e = hipMalloc(&dr, sizeof(int2) * nblocks);
                 if (e != hipSuccess) return fail(b, set_error(WS_ERR_NOMEM, "batch tile ranges"));
-                e = hipMemcpy(b->d_tile_ranges[shape], ranges.data(), sizeof(int2) * nblocks, hipMemcpyHostToDevice);
+                e = hipMemcpy(dr, ranges.data(), sizeof(int2) * nblocks, hipMemcpyHostToDevice);
                 if (e != hipSuccess) return fail(b, hip_error(e, "batch tile ranges upload"));
             }
             if (list.size() > (size_t)0x7fffffff) return fail(b, set_error(WS_ERR_INVALID, "too many tiles"));
@@ -308,7 +314,8 @@ int ws_batch_destroy(ws_batch *b)
     if (b->d_aux) hipFree(b->d_aux);
     if (b->d_cfl_part) hipFree(b->d_cfl_part);
     for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
-        if (b->d_tile_ranges[shape]) hipFree(b->d_tile_ranges[shape]);
+        for (int o = 0; o < WS_ORDER_COUNT; ++o)
+            if (b->d_tile_ranges[shape][o]) hipFree(b->d_tile_ranges[shape][o]);
     if (b->d_cfl_tiles) hipFree(b->d_cfl_tiles);
     if (b->capture_stream) hipStreamDestroy(b->capture_stream);
     delete b;
@@ -356,6 +363,21 @@ int ws_batch_set_stage_fusion(ws_batch *b, int32_t on)
     // 3 (the contexts' default: alternating primitive sets within ws_rk3_run) acts as 2 here
     if (on < 0 || on > 3) return set_error(WS_ERR_INVALID, "stage fusion must be 0, 1, 2 or 3");
     for (ws_ctx *c : b->ctx) c->fuse_stage = on;
+    return WS_OK;
+}
+
+int ws_batch_set_stage_order(ws_batch *b, int32_t order)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    if (order < 0 || order >= WS_ORDER_COUNT) return set_error(WS_ERR_INVALID, "unknown direction order");
+    b->stage_order = order;
+    return WS_OK;
+}
+
+int ws_batch_get_stage_order(const ws_batch *b, int32_t *order)
+{
+    if (!b || !order) return set_error(WS_ERR_INVALID, "null argument");
+    *order = b->stage_order;
     return WS_OK;
 }
 
@@ -416,8 +438,8 @@ int ws_batch_cfl_dt(ws_batch *b, double cfl, double *dt_dev, hipStream_t s)
 // cfl_stage_buffer; the alternating modes: its stage-3 launch, per tile in d_cfl_tiles).
 // mode and prepare as for a single box's rk3_step_fused (stage fusion 3): an alternating
 // step from set A runs stage 1 into B, stage 2 into A and stage 3 into B with the batched
-// z launch writing every block's primitives, sound speed and their periodic face halo;
-// the last step of a run from the second sets copies them to the blocks'
+// stage-applying launch writing every block's primitives, sound speed and their periodic
+// face halo; the last step of a run from the second sets copies them to the blocks'
 // fields and lets the stage-3 launch write the primitives and P over their interior.
 static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl_next, StepMode mode = STEP_PLAIN,
                       bool prepare = true)
@@ -426,7 +448,7 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
     for (ws_state &st : b->st) std::swap(st.f, st.fn);
     const ws_soa nolap{};
     int r = WS_OK;
-    // With the tiled kernels, the grid schedule and no viscous terms the batched z launch
+    // With the tiled kernels, the grid schedule and no viscous terms the last batched launch
     // applies each stage (as a single box's RK3 step does); the block table holds every
     // block's f and fn as created, and whether they are exchanged now is the same for all
     // blocks (they swap together).
@@ -435,7 +457,7 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
     const int swapped = b->st[0].f.ptr == b->fb0 ? 1 : 0;
     for (size_t i = 0; i < b->st.size() && fuse; ++i)
         fuse = b->st[i].f.ptr == (swapped ? b->fbs[i] : b->fas[i]);
-    // Fusion level 2 (as for a single box): the z launch of stages 1 and 2 also writes the
+    // Fusion level 2 (as for a single box): the last launch of stages 1 and 2 also writes the
     // primitives, stage 1 from the blocks' own fields into their second sets (d_blocks),
     // stage 2 back (d_blocks2), with the periodic face halo of the six written fields.
     const bool prims = fuse && b->ctx[0]->fuse_stage > 1 && prim_tables(b, s);
@@ -477,8 +499,8 @@ static int batch_step(ws_batch *b, const double *dt_dev, hipStream_t s, bool cfl
         return r;
     }
     // the blocks are periodic (ws_batch_create); stages 1 and 2 leave the next stage's
-    // inputs prepared (the z launch or the stage kernel writing the sound speed, the halo by
-    // the z launch itself or a ghost fill after the stage kernel)
+    // inputs prepared (the last launch or the stage kernel writing the sound speed, the halo
+    // by that launch itself or a ghost fill after the stage kernel)
     for (int krk = 1; krk <= 3 && !r; ++krk) {
         if (krk == 1) {
             if (aux) r = launch_prepare_batch(b->ctx[0]->kp, b->d_aux, nb, b->ghost_wgs, b->cell_wgs, s);
@@ -537,8 +559,9 @@ static bool batch_alternating(ws_batch *b, hipStream_t s, bool cfl)
     if (!cfl || b->d_cfl_tiles) return true;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
-    long n = 0;   // room for the z tile list of every shape
-    for (int shape = 0; shape < WS_TILE_COUNT; ++shape) n = std::max(n, (long)b->ntiles[shape][2]);
+    long n = 0;   // room for the x and z tile lists of every shape
+    for (int shape = 0; shape < WS_TILE_COUNT; ++shape)
+        n = std::max(n, (long)std::max(b->ntiles[shape][0], b->ntiles[shape][2]));
     if (hipMalloc(&b->d_cfl_tiles, sizeof(double) * n) != hipSuccess) {
         b->d_cfl_tiles = nullptr;
         (void)hipGetLastError();
@@ -571,7 +594,7 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
         // the previous step left its maxima per tile (alternating) or in the batch's buffer
         if (alt)
             return launch_cfl_batch(b->d_aux, nb, b->d_cfl_tiles, cfl, b->d_dts, dt_dev, ss,
-                                    b->d_tile_ranges[b->tile_shape]);
+                                    b->d_tile_ranges[b->tile_shape][b->stage_order]);
         if (b->cfl_in_batch) return launch_cfl_batch(b->d_aux, nb, b->d_cfl_part, cfl, b->d_dts, dt_dev, ss);
         for (size_t i = 0; i < b->ctx.size(); ++i) {
             ws_ctx *c = b->ctx[i];
@@ -632,6 +655,74 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
     return r;
 }
 
+// The batch's stage order, timed as ws_autotune times a context's: the batched RHS of an
+// RK3 stage 2 whose last launch writes every block's new f into scratch allocated here (a
+// copy of the block's f, which is the stage's fn and is only read) and, where the blocks
+// keep second primitive sets, the primitives, sound speed and their halo into those (the
+// sets a run's first stage overwrites anyway).  The faster order, or -1 when the scratch
+// cannot be had (the order then stays as it was).
+static int batch_tune_stage_order(ws_batch *b, int reps, hipEvent_t e0, hipEvent_t e1, hipStream_t s, int &r)
+{
+    const bool prims = b->ctx[0]->fuse_stage > 1 && prim_tables(b, s);
+    const size_t nb = b->st.size();
+    std::vector<size_t> off(nb), len(nb);   // each block's f in the scratch (doubles)
+    size_t total = 0;
+    for (size_t i = 0; i < nb; ++i) {
+        off[i] = total;
+        len[i] = 5 * (size_t)b->st[i].f.pitch * b->host_blocks[i].n[1] * b->host_blocks[i].n[2];
+        total += len[i];
+    }
+    double *scratch = nullptr;
+    BatchBlock *d_tab = nullptr;
+    if (hipMalloc(&scratch, total * sizeof(double)) != hipSuccess ||
+        hipMalloc(&d_tab, sizeof(BatchBlock) * nb) != hipSuccess) {
+        (void)hipGetLastError();
+        if (scratch) hipFree(scratch);
+        return -1;
+    }
+    // the block table with fa the scratch copy (the stage's f: BatchStage::swapped = 0) and
+    // fb the block's f; its outputs are the second sets once prim_tables has made them
+    std::vector<BatchBlock> tab = b->host_blocks;
+    for (size_t i = 0; i < nb && !r; ++i) {
+        const ws_soa f = b->st[i].f;
+        hipError_t e = hipMemcpyAsync(scratch + off[i], f.ptr, len[i] * sizeof(double), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) r = hip_error(e, "autotune stage scratch copy");
+        tab[i].fa = f;
+        tab[i].fa.ptr = scratch + off[i];
+        tab[i].fb = f;
+    }
+    if (!r) {
+        hipError_t e = hipMemcpy(d_tab, tab.data(), sizeof(BatchBlock) * nb, hipMemcpyHostToDevice);
+        if (e != hipSuccess) r = hip_error(e, "autotune stage table upload");
+    }
+    BatchStage bs{nullptr, b->base.dt, 2, 0};
+    bs.prims = prims ? 1 : 0;
+    bs.planes = b->base.bc_planes != 0;
+    bs.halo = bs.prims;
+    const int saved = b->stage_order;
+    int best = -1;
+    float best_ms = 1e30f;
+    for (int o = 0; o < WS_ORDER_COUNT && !r; ++o) {
+        b->stage_order = o;
+        for (int i = 0; i < 2 && !r; ++i) r = batch_rhs(b, s, &bs, d_tab);
+        if (r) break;
+        hipEventRecord(e0, s);
+        for (int i = 0; i < reps && !r; ++i) r = batch_rhs(b, s, &bs, d_tab);
+        hipEventRecord(e1, s);
+        if (r) break;
+        hipError_t e = hipEventSynchronize(e1);
+        if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) { best_ms = ms; best = o; }
+    }
+    b->stage_order = saved;
+    hipStreamSynchronize(s);
+    hipFree(d_tab);
+    hipFree(scratch);
+    return r ? -1 : best;
+}
+
 int ws_batch_autotune(ws_batch *b, int32_t reps, float *times_ms, hipStream_t s)
 {
     if (!b || reps <= 0) return set_error(WS_ERR_INVALID, "bad autotune request");
@@ -662,6 +753,12 @@ int ws_batch_autotune(ws_batch *b, int32_t reps, float *times_ms, hipStream_t s)
             if (times_ms) times_ms[shape * WS_SCHED_COUNT + sched] = ms;
             if (ms < best_ms) { best_ms = ms; best_shape = shape; best_sched = sched; }
         }
+    if (!r && best_sched == WS_SCHED_GRID && !b->base.viscous && b->ctx[0]->fuse_stage) {
+        b->tile_shape = best_shape;
+        b->schedule = best_sched;
+        const int o = batch_tune_stage_order(b, reps, e0, e1, s, r);
+        if (o >= 0) b->stage_order = o;
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     b->tile_shape = r ? saved_shape : best_shape;

@@ -864,10 +864,29 @@ void launch_batch_dir(const KParams &k, const BatchTiles &bt, hipStream_t s)
                        k, bt.blocks, bt.tiles[DIR], n, chunk);
 }
 
+// one tile per work-group over direction DIR's tile list (the grid schedule), STG: this
+// launch applies the RK3 stage bt.stage
+template <int DIR, int L, int W, int K, bool ACC, bool MIX, bool STG>
+void launch_batch_grid(const KParams &k, const BatchTiles &bt, hipStream_t s)
+{
+    const int n = bt.ntiles[DIR];
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_rhs_batch_grid<DIR, L, W, K, ACC, MIX, STG>), dim3(8 * ((n + 7) / 8)),
+                       dim3(Tile<DIR, L, W, K>::NT), 0, s, k, bt.blocks, bt.tiles[DIR], n,
+                       STG ? *bt.stage : BatchStage{});
+}
+
 template <int S, bool MIX, bool PF>
 void launch_batch_shape_n(const KParams &k, const BatchTiles &bt, hipStream_t s)
 {
     constexpr TileShapeDef D = kTileShapes[S];
+    if (bt.stage && bt.stage_order == WS_ORDER_ZYX) {
+        // z writes du, y accumulates, x adds its terms and applies the stage
+        launch_batch_grid<2, D.L2, D.W2, D.K, false, MIX, false>(k, bt, s);
+        launch_batch_grid<1, D.L1, D.W1, D.K, true, MIX, false>(k, bt, s);
+        launch_batch_grid<0, D.L0, D.W0, D.K, true, MIX, true>(k, bt, s);
+        return;
+    }
     launch_batch_dir<0, D.L0, D.W0, D.K, false, MIX, PF>(k, bt, s);
     launch_batch_dir<1, D.L1, D.W1, D.K, true, MIX, PF>(k, bt, s);
     launch_batch_dir<2, D.L2, D.W2, D.K, true, MIX, PF>(k, bt, s);

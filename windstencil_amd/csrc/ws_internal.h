@@ -150,7 +150,8 @@ struct BatchTiles {
     const BatchBlock *blocks;
     const int4 *tiles[3];
     int ntiles[3];
-    const BatchStage *stage;   // non-null: the z launch applies this RK3 stage (grid schedule)
+    const BatchStage *stage;   // non-null: the last launch applies this RK3 stage (grid schedule)
+    int stage_order;           // with stage: WS_ORDER_XYZ (z applies it) or WS_ORDER_ZYX (x does)
 };
 
 // One block of a batch for the batched helper kernels (ghost fill, sound speed, the stage
