@@ -168,10 +168,29 @@ __device__ __forceinline__ void tile_cell_clamped(const TileGeo &g, int c, int &
 template <int DIR, int L, int W, int K>
 __device__ __forceinline__ void tile_load(int nyg, const ws_field &d, const ws_field &u, const ws_field &v,
                                           const ws_field &w, const ws_field &cc, const TileGeo &g,
-                                          TileLoads<DIR, L, W, K> &ld)
+                                          TileLoads<DIR, L, W, K> &ld, int off32)
 {
     using TL = Tile<DIR, L, W, K>;
     const int lhi = min(g.lim_across, W) - 1, phi = min(g.lim_line + 2 * LAP, TL::NL) - 1;
+    if (off32) {
+        // one 32-bit offset for the five fields (KParams::off32: a uniform branch)
+#pragma unroll
+        for (int r = 0; r < TL::ROUNDS; ++r) {
+            int l, p, x, y, z;
+            tile_point_at<DIR, L, W, K>(min((int)threadIdx.x + r * TL::NT, TL::NPTS - 1), l, p);
+            l = min(max(l, g.lo_across), lhi);
+            p = min(p, phi);
+            tile_point_xyz<DIR>(g, l, p, x, y, z);
+            const unsigned o = (unsigned)x + __umul24(d.pitch, (unsigned)y + __umul24((unsigned)nyg, (unsigned)z));
+            ld.v[r][0] = d.ptr[o];
+            ld.v[r][1] = u.ptr[o];
+            ld.v[r][2] = v.ptr[o];
+            ld.v[r][3] = w.ptr[o];
+            ld.v[r][4] = cc.ptr[o];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < TL::ROUNDS; ++r) {
         int l, p, x, y, z;
@@ -541,7 +560,7 @@ k_rhs_tile(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_field c
     const int r = tile / gx;
     const TileGeo g = tile_geo<DIR, L, W, K>(box, tile - r * gx, r % gy, r / gy);
     TileLoads<DIR, L, W, K> ld;
-    tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
+    tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld, k.off32);
     const double cmax = tile_compute<DIR, L, W, K, FIELDS, ACC, MIX, BND, STG>(k, mf, du, g, ld, sp, sa, [] {});
     if (STG && sa.cfl_part) {
         __syncthreads();   // phase 3 has read its faces from sp
@@ -570,13 +589,13 @@ k_rhs_persist(KParams k, ws_field d, ws_field u, ws_field v, ws_field w, ws_fiel
     };
     TileLoads<DIR, L, W, K> ld;
     TileGeo g = geo(first);
-    if (PREFETCH) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
+    if (PREFETCH) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld, k.off32);
     for (int i = first; i < last; ++i) {
-        if (!PREFETCH) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld);
+        if (!PREFETCH) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, g, ld, k.off32);
         const bool more = i + 1 < last;
         const TileGeo gn = more ? geo(i + 1) : g;
         auto next = [&] {
-            if (PREFETCH && more) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, gn, ld);
+            if (PREFETCH && more) tile_load<DIR, L, W, K>(k.nyg, d, u, v, w, cc, gn, ld, k.off32);
         };
         tile_compute<DIR, L, W, K, false, ACC, MIX, false, false>(k, nomf, du, g, ld, sp, StageArgs{}, next);
         __syncthreads();   // the next tile reuses the LDS buffer
@@ -617,7 +636,7 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
     TileGeo g = tile_geo<DIR, L, W, K>(box_of(blocks[tl.x]), tl.y, tl.z, tl.w);
     if (PREFETCH) {
         const BatchBlock &b = blocks[tl.x];
-        tile_load<DIR, L, W, K>(b.n[1] + 2 * LAP, b.d, b.u, b.v, b.w, b.cc, g, ld);
+        tile_load<DIR, L, W, K>(b.n[1] + 2 * LAP, b.d, b.u, b.v, b.w, b.cc, g, ld, k.off32);
     }
     for (int i = first; i < last; ++i) {
         const BatchBlock &b = blocks[tl.x];
@@ -625,7 +644,7 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
         kb.nx = b.n[0]; kb.ny = b.n[1]; kb.nz = b.n[2];
         kb.nyg = b.n[1] + 2 * LAP; kb.nzg = b.n[2] + 2 * LAP;
         kb.rh[DIR] = b.rh[DIR];
-        if (!PREFETCH) tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
+        if (!PREFETCH) tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld, k.off32);
         const bool more = i + 1 < last;
         const int4 tn = more ? tile_at(i + 1) : tl;
         const TileGeo gn = tile_geo<DIR, L, W, K>(box_of(blocks[tn.x]), tn.y, tn.z, tn.w);
@@ -634,7 +653,7 @@ k_rhs_batch(KParams k, const BatchBlock *blocks, const int4 *tiles, int ntiles, 
         auto next = [&] {
             if (PREFETCH && more) {
                 const BatchBlock &bn = blocks[tn.x];
-                tile_load<DIR, L, W, K>(bn.n[1] + 2 * LAP, bn.d, bn.u, bn.v, bn.w, bn.cc, gn, ld);
+                tile_load<DIR, L, W, K>(bn.n[1] + 2 * LAP, bn.d, bn.u, bn.v, bn.w, bn.cc, gn, ld, k.off32);
             }
         };
         tile_compute<DIR, L, W, K, false, ACC, MIX, false, false>(kb, nomf, b.du, g, ld, sp, StageArgs{}, next);
@@ -667,7 +686,7 @@ k_rhs_batch_grid(KParams k, const BatchBlock *blocks, const int4 *tiles, int nti
     box.e[0] = b.n[0] + LAP; box.e[1] = b.n[1] + LAP; box.e[2] = b.n[2] + LAP;
     const TileGeo g = tile_geo<DIR, L, W, K>(box, tl.y, tl.z, tl.w);
     TileLoads<DIR, L, W, K> ld;
-    tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld);
+    tile_load<DIR, L, W, K>(kb.nyg, b.d, b.u, b.v, b.w, b.cc, g, ld, k.off32);
     const ws_metric_fields nomf{};
     StageArgs sa{};
     if (STG) {
@@ -915,12 +934,13 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
     ws_metric_fields mf{};
     const int shape = c.tile_shape;
     const int order = sa ? c.stage_order : c.dir_order;
+    KParams k = c.kp;
+    k.off32 = prim_off32(k, d, u, v, w, cc) ? 1 : 0;
     if (m) {
         mf = *m;
-        launch_shape<true, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, order, nullptr,
-                                s);
+        launch_shape<true, MIX>(shape, k, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, order, nullptr, s);
     } else {
-        launch_shape<false, MIX>(shape, c.kp, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, order, sa, s);
+        launch_shape<false, MIX>(shape, k, d, u, v, w, cc, mf, du, box, accumulate, c.schedule, order, sa, s);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, MIX ? "mixed-precision RHS launch" : "tiled RHS launch");

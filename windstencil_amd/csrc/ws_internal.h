@@ -37,7 +37,21 @@ struct KParams {
     int bnd[6];
     int nonref[6];
     int bnd_any[3];
+    // set per launch: the RHS inputs d, u, v, w, cc share d's pitch and every offset into
+    // them is below 2^31 with pitch and nyg*nzg below 2^24, so one 32-bit offset with
+    // 24-bit products addresses all five (prim_off32)
+    int off32 = 0;
 };
+
+// whether the five RHS input fields of a box of these extents can be addressed with one
+// 32-bit offset (KParams::off32)
+inline bool prim_off32(const KParams &k, const ws_field &d, const ws_field &u, const ws_field &v,
+                       const ws_field &w, const ws_field &cc)
+{
+    const unsigned long long p = d.pitch, plane = (unsigned long long)k.nyg * (unsigned long long)k.nzg;
+    return u.pitch == d.pitch && v.pitch == d.pitch && w.pitch == d.pitch && cc.pitch == d.pitch &&
+           p < (1ull << 24) && plane < (1ull << 24) && p * plane < (1ull << 31);
+}
 
 // The SSP-RK3 stage the last directional RHS launch applies to its cells itself (f from fn,
 // f and the complete RHS, which then never goes through du): stage krk (1..3), dt from
