@@ -32,6 +32,9 @@ struct ws_batch {
     // which launch of a stage-applying RHS applies the stage (WS_ORDER_*: the last
     // direction; ws_batch_autotune picks)
     int stage_order = WS_ORDER_XYZ;
+    // KParams::off32 for the RHS inputs of d_blocks (every block's own fields) and of
+    // d_blocks2 (the second primitive sets): every block's five fields pass prim_off32
+    int off32_own = 0, off32_second = 0;
     std::vector<ws_ctx *> ctx;
     std::vector<ws_state> st;
     BatchBlock *d_blocks = nullptr;
@@ -111,7 +114,8 @@ int batch_rhs(ws_batch *b, hipStream_t s, const BatchStage *stage = nullptr, con
         }
         return WS_OK;
     }
-    const KParams &k = b->ctx[0]->kp;   // physics constants and metrics are the batch's
+    KParams k = b->ctx[0]->kp;   // physics constants and metrics are the batch's
+    k.off32 = table && table == b->d_blocks2 ? b->off32_second : b->off32_own;
     BatchTiles t = tiles_of(b);
     t.stage = stage;
     if (table) t.blocks = table;
@@ -128,9 +132,11 @@ bool prim_tables(ws_batch *b, hipStream_t s)
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
     const size_t n = b->host_blocks.size();
     std::vector<BatchBlock> a = b->host_blocks, c = b->host_blocks;
+    int second32 = 1;
     for (size_t i = 0; i < n; ++i) {
         ws_field sec[6];
         if (!prim_set(b->ctx[i], s, sec)) return false;
+        if (!prim_off32(b->ctx[i]->kp, sec[0], sec[1], sec[2], sec[3], sec[5])) second32 = 0;
         const ws_state &st = b->st[i];
         a[i].od = sec[0]; a[i].ou = sec[1]; a[i].ov = sec[2]; a[i].ow = sec[3]; a[i].oT = sec[4]; a[i].occ = sec[5];
         c[i].d = sec[0]; c[i].u = sec[1]; c[i].v = sec[2]; c[i].w = sec[3]; c[i].cc = sec[5];
@@ -146,6 +152,7 @@ bool prim_tables(ws_batch *b, hipStream_t s)
     }
     b->host_blocks = a;
     b->d_blocks2 = d2;
+    b->off32_second = second32;
     return true;
 }
 
@@ -259,6 +266,11 @@ int ws_batch_create(const ws_config *base, int32_t nblocks, const int32_t *exten
         h.n[0] = cfg.nx; h.n[1] = cfg.ny; h.n[2] = cfg.nz;
         h.rh[0] = 1.0 / cfg.hx; h.rh[1] = 1.0 / cfg.hy; h.rh[2] = 1.0 / cfg.hz;
         b->cells += (long long)cfg.nx * cfg.ny * cfg.nz;
+    }
+    b->off32_own = 1;
+    for (int i = 0; i < nblocks; ++i) {
+        const ws_state &st = states[i];
+        if (!prim_off32(b->ctx[i]->kp, st.d, st.u, st.v, st.w, st.cc)) b->off32_own = 0;
     }
     hipError_t e = hipMalloc(&b->d_blocks, sizeof(BatchBlock) * nblocks);
     if (e != hipSuccess) return fail(b, set_error(WS_ERR_NOMEM, "batch block table"));
