@@ -189,6 +189,26 @@ def test_tiled_is_independent_of_pitch(ws):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+@pytest.mark.parametrize("tile", W.TILE_SHAPES)
+def test_inputs_of_different_pitches_take_the_64_bit_addresses(ws, tile, precision):
+    """The load rounds address the five input fields with one 32-bit offset when they share
+    a pitch; a sound-speed field of another pitch makes the launch address each field on its
+    own in 64 bits.  Both address the same cells, so the RHS agrees bit for bit, for every
+    tile shape and both precisions."""
+    nx, ny, nz = 45, 26, 19
+    prims = random_state(nx, ny, nz, 17, "shock")
+    ctx = make_ctx(nx, ny, nz, precision=precision)
+    ctx.set_tile_shape(tile)
+    a = gpu_du_comput(ctx, upload_state(prims))
+    st = upload_state(prims)
+    st.t["cc"] = W.ghosted_tensor(nx, ny, nz, pad=5)
+    st.refresh()
+    assert st.ws.cc.pitch != st.ws.d.pitch
+    b = gpu_du_comput(ctx, st)
+    assert np.array_equal(a, b)
+
+
 def test_metric_fields(orc, ws):
     """Per-cell metric arrays (all terms 1, Ajac 0.5: the reference's Init_stat = 0
     metrics) through both kernel families."""
