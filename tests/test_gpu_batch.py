@@ -265,6 +265,44 @@ def test_batch_graph_replay_equals_eager(ws, nsteps, schedule):
         assert np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+def test_batch_with_a_block_of_different_pitches_takes_the_64_bit_addresses(ws, precision):
+    """A batch addresses its blocks' five input fields with one 32-bit offset only when every
+    block's fields share a pitch; one block whose sound speed has another pitch makes the
+    whole batch address each field in 64 bits.  The RHS and three RK3 steps (stage fusion 3,
+    CFL step) agree bit for bit with the batch of common pitches."""
+    extents = _extents(31, 4, 6, 30)
+    kinds = [k for k in KINDS if k != "rough"]
+    runs = []
+    for odd in (False, True):
+        prims = [random_state(*e, 900 + i, kinds[i % len(kinds)]) for i, e in enumerate(extents)]
+        states = [upload_state(p) for p in prims]
+        if odd:
+            nx, ny, nz = extents[1]
+            states[1].t["cc"] = W.ghosted_tensor(nx, ny, nz, pad=3)
+            states[1].refresh()
+            assert states[1].ws.cc.pitch != states[1].ws.d.pitch
+        b = W.Batch(W.default_config(8, 8, 8, precision=precision, bc_planes=1, dt=2e-4), extents, states)
+        W.check(W.lib().ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
+        for e, st in zip(extents, b.states):
+            _to_cons(make_ctx(*e), st)
+        s = W.stream_handle()
+        W.check(W.lib().ws_batch_du_comput(b.handle, s))
+        _sync()
+        du = [_du(st) for st in b.states]
+        W.check(W.lib().ws_batch_rk3_run(b.handle, 3, 0.5, 0, s))
+        b.sync_states()
+        _sync()
+        runs.append((du, [{k: st.interior(k).cpu().numpy() for k in ("f", "d", "u", "v", "w", "T")}
+                          for st in b.states]))
+        b.close()
+    (du_a, st_a), (du_b, st_b) = runs
+    for i in range(len(extents)):
+        assert np.array_equal(du_a[i], du_b[i]), i
+        for k in st_a[i]:
+            assert np.array_equal(st_a[i][k], st_b[i][k]), (i, k)
+
+
 def test_batch_autotune_keeps_results(ws):
     """The tuned batch gives the same RHS to rounding; tuning (the stage order's timing
     included, when the grid schedule wins) leaves every block's f and primitives as they
