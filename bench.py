@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--order", type=int, default=-1,
                     help="rk3 and batched: force the stage order after tuning (0 x, y, z: z applies "
                          "the stage; 1 z, y, x: x applies it)")
+    ap.add_argument("--stage-shape", type=int, default=-2,
+                    help="rk3: force the tile shape of the stage-applying launches after tuning "
+                         "(-1: the context's shape)")
     ap.add_argument("--workload", default="rhs", choices=["rhs", "batched", "rk3"],
                     help="rhs: the headline RHS line (default); batched: 48 uneven blocks; "
                          "rk3: SSP-RK3 steps of one box in a graph")
@@ -322,15 +325,18 @@ def bench_batched(args):
 
 
 def _ctx_choice(ctx):
-    """'shape/schedule/order/stage order' the context runs its tiled RHS with."""
+    """'shape/schedule/order/stage order/stage shape' the context runs its tiled RHS with
+    (stage shape -1: the context's shape)."""
     import windstencil_amd as W
     L = W.lib()
     chosen, sched, order, sorder = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    sshape = ctypes.c_int32()
+    W.check(L.ws_ctx_get_stage_shape(ctx.handle, ctypes.byref(sshape)))
     W.check(L.ws_ctx_get_tile_shape(ctx.handle, ctypes.byref(chosen)))
     W.check(L.ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
     W.check(L.ws_ctx_get_direction_order(ctx.handle, ctypes.byref(order)))
     W.check(L.ws_ctx_get_stage_order(ctx.handle, ctypes.byref(sorder)))
-    return f"{chosen.value}/{sched.value}/{order.value}/{sorder.value}"
+    return f"{chosen.value}/{sched.value}/{order.value}/{sorder.value}/{sshape.value}"
 
 
 def bench_rk3(args):
@@ -366,6 +372,8 @@ def bench_rk3(args):
         W.check(L.ws_autotune(ctx.handle, ctypes.byref(st.ws), 3, None, sh))
     if args.order >= 0:
         W.check(L.ws_ctx_set_stage_order(ctx.handle, args.order))
+    if args.stage_shape >= -1:
+        W.check(L.ws_ctx_set_stage_shape(ctx.handle, args.stage_shape))
     W.check(L.ws_rk3_run(ctx.handle, ctypes.byref(st.ws), 2, 0.5, 1, sh))     # warm-up steps
     st.sync_from_ws()
     torch.cuda.synchronize()
@@ -390,7 +398,7 @@ def bench_rk3(args):
                    "cells": N},
                   roofline=roof, rk3_steps=nsteps, seconds=round(secs, 4), state_finite=finite,
                   mcell_steps_per_s=round(N * nsteps / secs / 1e6, 2),
-                  stage_fusion=args.stage_fusion, **{"tile_shape/schedule/order/stage_order": _ctx_choice(ctx)},
+                  stage_fusion=args.stage_fusion, **{"tile_shape/schedule/order/stage_order/stage_shape": _ctx_choice(ctx)},
                   totals=[float(v) for v in f.sum(dim=(1, 2, 3)).cpu()])
     out["steps"] = nsteps
     print(json.dumps(out))

@@ -196,6 +196,12 @@ int ws_ctx_get_direction_order(const ws_ctx *ctx, int32_t *order);
  * to rounding otherwise.  No counterpart in the reference. */
 int ws_ctx_set_stage_order(ws_ctx *ctx, int32_t order);
 int ws_ctx_get_stage_order(const ws_ctx *ctx, int32_t *order);
+/* WS_TILE_* of those stage-applying launches, or -1 (the default): the context's tile
+ * shape.  Chosen by the autotuner with the stage order; the same bit-for-bit and rounding
+ * statements hold with "tile shape" for "direction order".  No counterpart in the
+ * reference. */
+int ws_ctx_set_stage_shape(ws_ctx *ctx, int32_t shape);
+int ws_ctx_get_stage_shape(const ws_ctx *ctx, int32_t *shape);
 /* 2: RK3 steps let the last directional RHS launch apply each stage's update (tiled
  * kernels, grid schedule, constant metrics, no closures, no viscous terms; same result as
  * the separate stage kernel, du then holds partial sums) and, with every direction
@@ -285,9 +291,10 @@ int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipSt
  * precision, on this state (two untimed and `reps` timed RHS launches each, HIP events on
  * `stream`), keep the fastest for the context and remember it for contexts of the same
  * extents and precision.  When RK3 steps of that choice apply their stages in the last RHS
- * launch (ws_ctx_set_stage_fusion), the stage order is then timed the same way: the RHS of
- * an RK3 stage 2 whose last launch writes the new f, primitives, sound speed and halo into
- * scratch the call allocates and frees (the state's f is read, not written).  times_ms
+ * launch (ws_ctx_set_stage_fusion), the stage order and tile shape are then timed the same
+ * way (every shape in both orders): the RHS of an RK3 stage 2 whose last launch writes the
+ * new f, primitives, sound speed and halo into scratch the call allocates and frees (the
+ * state's f is read, not written).  times_ms
  * (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major, each the faster of the two orders)
  * may be NULL.  Overwrites st->du; synchronises the stream.  No counterpart in the
  * reference. */

@@ -101,17 +101,23 @@ def test_tile_shapes_agree_and_autotune_keeps_parity(orc, ws):
     W.check(W.lib().ws_ctx_get_schedule(ctx.handle, ctypes.byref(sched)))
     W.check(W.lib().ws_ctx_get_direction_order(ctx.handle, ctypes.byref(order)))
     W.check(W.lib().ws_ctx_get_stage_order(ctx.handle, ctypes.byref(sorder)))
-    log(f"  chose shape {chosen.value} schedule {sched.value} order {order.value} stage order {sorder.value}")
+    sshape = ctypes.c_int32()
+    W.check(W.lib().ws_ctx_get_stage_shape(ctx.handle, ctypes.byref(sshape)))
+    log(f"  chose shape {chosen.value} schedule {sched.value} order {order.value} stage order {sorder.value}"
+        f" stage shape {sshape.value}")
     assert times[chosen.value * ns + sched.value] == min(times)
     assert order.value in W.ORDERS and sorder.value in W.ORDERS
+    assert sched.value != W.WS_SCHED_GRID or sshape.value in W.TILE_SHAPES
     ctx2 = make_ctx(nx, ny, nz)
     again, again_s, again_o, again_so = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     W.check(W.lib().ws_ctx_get_tile_shape(ctx2.handle, ctypes.byref(again)))
     W.check(W.lib().ws_ctx_get_schedule(ctx2.handle, ctypes.byref(again_s)))
     W.check(W.lib().ws_ctx_get_direction_order(ctx2.handle, ctypes.byref(again_o)))
     W.check(W.lib().ws_ctx_get_stage_order(ctx2.handle, ctypes.byref(again_so)))
-    assert (again.value, again_s.value, again_o.value, again_so.value) == \
-        (chosen.value, sched.value, order.value, sorder.value)
+    again_ss = ctypes.c_int32()
+    W.check(W.lib().ws_ctx_get_stage_shape(ctx2.handle, ctypes.byref(again_ss)))
+    assert (again.value, again_s.value, again_o.value, again_so.value, again_ss.value) == \
+        (chosen.value, sched.value, order.value, sorder.value, sshape.value)
     assert rel_l2(gpu_du_comput(ctx2, upload_state(prims)), ref) <= TOL_FP64
 
 

@@ -124,7 +124,7 @@ def test_time_advance_stages_bitwise(orc, ws):
 
 
 def _gpu_run(prims, nsteps, kernel, graph, cfl=0.5, precision=W.WS_PREC_FP64, order=None, fusion=None,
-             stage_order=None):
+             stage_order=None, tile=None, stage_shape=None):
     nz, ny, nx = prims[0].shape
     L = W.lib()
     s = W.stream_handle()
@@ -134,6 +134,10 @@ def _gpu_run(prims, nsteps, kernel, graph, cfl=0.5, precision=W.WS_PREC_FP64, or
         ctx.set_stage_order(order if stage_order is None else stage_order)
     elif stage_order is not None:
         ctx.set_stage_order(stage_order)
+    if tile is not None:
+        ctx.set_tile_shape(tile)
+    if stage_shape is not None:
+        ctx.set_stage_shape(stage_shape)
     if fusion is not None:
         ctx.set_stage_fusion(fusion)
     st = upload_state(prims)
@@ -218,6 +222,24 @@ def test_stage_applying_launches_follow_the_stage_order(ws, precision, stage_ord
     for k in ("d", "u", "v", "w", "T", "P", "cc"):
         assert np.array_equal(sa.t[k].cpu().numpy(), sb.t[k].cpu().numpy()), k
     assert 0 < err <= (1e-12 if precision == W.WS_PREC_FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("precision", [W.WS_PREC_FP64, W.WS_PREC_MIXED])
+def test_stage_applying_launches_follow_the_stage_shape(ws, precision):
+    """The stage-applying RHS launches run in the stage shape when one is set: fusion 3
+    with tile shape 1 and stage shape 4 equals the separate stage kernels with tile shape 4
+    bit for bit (every ghosted primitive field included), and stage shape -1 follows the
+    tile shape."""
+    prims = random_state(47, 26, 19, 13, "shock")
+    a, sa = _gpu_run(prims, 3, W.WS_KERNEL_TILED, graph=False, precision=precision, order=W.WS_ORDER_ZYX,
+                     fusion=3, tile=W.WS_TILE_30x16_K2, stage_shape=W.WS_TILE_126x2_K2)
+    b, sb = _gpu_run(prims, 3, W.WS_KERNEL_TILED, graph=False, precision=precision, order=W.WS_ORDER_ZYX,
+                     fusion=0, tile=W.WS_TILE_126x2_K2)
+    c, _ = _gpu_run(prims, 3, W.WS_KERNEL_TILED, graph=False, precision=precision, order=W.WS_ORDER_ZYX,
+                    fusion=3, tile=W.WS_TILE_126x2_K2, stage_shape=-1)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    for k in ("d", "u", "v", "w", "T", "P", "cc"):
+        assert np.array_equal(sa.t[k].cpu().numpy(), sb.t[k].cpu().numpy()), k
 
 
 @pytest.mark.parametrize("nsteps", [1, 2, 3, 5])

@@ -97,6 +97,8 @@ _SIGS = {
     "ws_ctx_get_direction_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_ctx_set_stage_order": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_ctx_get_stage_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_ctx_set_stage_shape": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_ctx_get_stage_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_ctx_set_stage_fusion": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_state_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
     "ws_state_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(WsState)]),
@@ -273,6 +275,10 @@ class Context:
     def set_stage_order(self, o):
         """Direction order of the RHS launches that apply an RK3 stage (WS_ORDER_*)."""
         check(lib().ws_ctx_set_stage_order(self.handle, o), "ws_ctx_set_stage_order")
+
+    def set_stage_shape(self, s):
+        """Tile shape of the RHS launches that apply an RK3 stage (-1: the context's)."""
+        check(lib().ws_ctx_set_stage_shape(self.handle, s), "ws_ctx_set_stage_shape")
 
     def set_stage_fusion(self, level):
         """0: separate stage kernel; 1: the last RHS launch applies each stage; 2: it also

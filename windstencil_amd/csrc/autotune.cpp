@@ -3,10 +3,11 @@
 // Every (tile shape, schedule, direction order) runs the full inviscid RHS on the caller's
 // state, in the context's precision: two untimed launches, then `reps` launches bracketed
 // by HIP events on the caller's stream.  The fastest becomes the context's.  When RK3
-// steps of that choice let the last RHS launch apply the stage, both direction orders of
-// that stage-applying RHS are timed the same way (into scratch outputs) and the faster
-// becomes the stage order.  The choice is remembered for the grid extents and precision,
-// so later contexts of the same size and precision start from it.
+// steps of that choice let the last RHS launch apply the stage, that stage-applying RHS is
+// timed the same way in every tile shape and both direction orders (into scratch outputs)
+// and the fastest becomes the stage shape and order.  The choice is remembered for the
+// grid extents and precision, so later contexts of the same size and precision start from
+// it.
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -22,7 +23,7 @@ namespace {
 
 std::mutex g_mu;
 struct Choice {
-    int shape, schedule, order, stage_order;
+    int shape, schedule, order, stage_order, stage_shape;
 };
 // (nx, ny, nz, precision) -> choice
 std::map<std::tuple<int, int, int, int>, Choice> g_best;
@@ -54,38 +55,42 @@ float time_launches(F fn, int reps, hipEvent_t e0, hipEvent_t e1, hipStream_t s,
     return ms / (float)reps;
 }
 
-// The stage order: the RHS of an RK3 stage 2 (f = 3/4 fn + 1/4 (f + dt RHS)) whose last
-// launch writes the new f and, with every direction periodic, the primitives, sound speed,
-// their periodic halo and the per-tile CFL maxima: what a fused step's stage launches
-// write.  fn is the state's f (read only); f, the primitives and the maxima go to scratch
-// allocated here (f a copy of the state's, so the values stay physical).  Returns the
-// faster order, or -1 when the scratch cannot be had (the order then stays as it was).
-int tune_stage_order(ws_ctx *c, const ws_state *st, const ws_job &job, int reps, hipEvent_t e0, hipEvent_t e1,
-                     hipStream_t s, int &r)
+// The stage shape and order: the RHS of an RK3 stage 2 (f = 3/4 fn + 1/4 (f + dt RHS))
+// whose last launch writes the new f and, with every direction periodic, the primitives,
+// sound speed, their periodic halo and the per-tile CFL maxima: what a fused step's stage
+// launches write.  fn is the state's f (read only); f, the primitives and the maxima go to
+// scratch allocated here (f a copy of the state's, so the values stay physical).  Sets
+// *shape and *order to the fastest; false when the scratch cannot be had (both then stay
+// as they were) or on an error (r set).
+bool tune_stage(ws_ctx *c, const ws_state *st, const ws_job &job, int reps, hipEvent_t e0, hipEvent_t e1,
+                hipStream_t s, int &r, int *shape_out, int *order_out)
 {
     const bool periodic = c->cfg.periodic[0] && c->cfg.periodic[1] && c->cfg.periodic[2];
     const bool prims = periodic && c->fuse_stage > 1;
     const size_t fper = (size_t)st->f.pitch * c->kp.ny * c->kp.nz;
     const size_t pper = (size_t)st->d.pitch * c->kp.nyg * c->kp.nzg;
-    const int saved = c->stage_order;
+    const int saved = c->stage_order, saved_shape = c->stage_shape;
     long tiles = 0;
-    for (int o = 0; o < WS_ORDER_COUNT; ++o) {
-        c->stage_order = o;
-        const long n = stage_launch_tiles(*c);
-        if (n > tiles) tiles = n;
-    }
+    for (int sh = 0; sh < WS_TILE_COUNT; ++sh)
+        for (int o = 0; o < WS_ORDER_COUNT; ++o) {
+            c->stage_shape = sh;
+            c->stage_order = o;
+            const long n = stage_launch_tiles(*c);
+            if (n > tiles) tiles = n;
+        }
     c->stage_order = saved;
+    c->stage_shape = saved_shape;
     const size_t elems = 5 * fper + (prims ? 6 * pper + (size_t)tiles : 0);
     double *scratch = nullptr;
     if (hipMalloc(&scratch, elems * sizeof(double)) != hipSuccess) {
         (void)hipGetLastError();
-        return -1;
+        return false;
     }
     hipError_t e = hipMemcpyAsync(scratch, st->f.ptr, 5 * fper * sizeof(double), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) {
         r = hip_error(e, "autotune stage scratch copy");
         hipFree(scratch);
-        return -1;
+        return false;
     }
     ws_soa f = st->f;
     f.ptr = scratch;
@@ -102,20 +107,23 @@ int tune_stage_order(ws_ctx *c, const ws_state *st, const ws_job &job, int reps,
                    ? launch_rhs_mixed_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, job, sa, s)
                    : launch_rhs_tiled_stage(*c, st->d, st->u, st->v, st->w, st->cc, st->du, job, sa, s);
     };
-    int best = -1;
     float best_ms = 1e30f;
-    for (int o = 0; o < WS_ORDER_COUNT && !r; ++o) {
-        c->stage_order = o;
-        const float ms = time_launches(stage, reps, e0, e1, s, r);
-        if (!r && ms < best_ms) {
-            best_ms = ms;
-            best = o;
+    for (int sh = 0; sh < WS_TILE_COUNT && !r; ++sh)
+        for (int o = 0; o < WS_ORDER_COUNT && !r; ++o) {
+            c->stage_shape = sh;
+            c->stage_order = o;
+            const float ms = time_launches(stage, reps, e0, e1, s, r);
+            if (!r && ms < best_ms) {
+                best_ms = ms;
+                *shape_out = sh;
+                *order_out = o;
+            }
         }
-    }
     c->stage_order = saved;
+    c->stage_shape = saved_shape;
     hipStreamSynchronize(s);
     hipFree(scratch);
-    return r ? -1 : best;
+    return !r;
 }
 
 }  // namespace
@@ -123,7 +131,7 @@ int tune_stage_order(ws_ctx *c, const ws_state *st, const ws_job &job, int reps,
 namespace ws {
 
 bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order,
-                       int *stage_order)
+                       int *stage_order, int *stage_shape)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_best.find(std::make_tuple(nx, ny, nz, precision));
@@ -132,6 +140,7 @@ bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *s
     *schedule = it->second.schedule;
     *order = it->second.order;
     *stage_order = it->second.stage_order;
+    *stage_shape = it->second.stage_shape;
     return true;
 }
 
@@ -148,7 +157,7 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
     if (e != hipSuccess) return hip_error(e, "autotune event");
     e = hipEventCreate(&e1);
     if (e != hipSuccess) { hipEventDestroy(e0); return hip_error(e, "autotune event"); }
-    const Choice saved{c->tile_shape, c->schedule, c->dir_order, c->stage_order};
+    const Choice saved{c->tile_shape, c->schedule, c->dir_order, c->stage_order, c->stage_shape};
     Choice best = saved;
     int r = WS_OK;
     float best_ms = 1e30f;
@@ -163,7 +172,10 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
                 const float ms = time_launches(rhs, reps, e0, e1, s, r);
                 if (r) break;
                 if (ms < pair_ms) pair_ms = ms;
-                if (ms < best_ms) { best_ms = ms; best = Choice{shape, sched, order, saved.stage_order}; }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = Choice{shape, sched, order, saved.stage_order, saved.stage_shape};
+                }
             }
             if (times_ms) times_ms[shape * WS_SCHED_COUNT + sched] = pair_ms;
         }
@@ -171,9 +183,10 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
         c->tile_shape = best.shape;
         c->schedule = best.schedule;
         c->dir_order = best.order;
-        if (fused_stage_ok(c)) {
-            const int o = tune_stage_order(c, st, job, reps, e0, e1, s, r);
-            if (o >= 0) best.stage_order = o;
+        int sh = best.stage_shape, o = best.stage_order;
+        if (fused_stage_ok(c) && tune_stage(c, st, job, reps, e0, e1, s, r, &sh, &o)) {
+            best.stage_shape = sh;
+            best.stage_order = o;
         }
     }
     hipEventDestroy(e0);
@@ -183,6 +196,7 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
     c->schedule = use.schedule;
     c->dir_order = use.order;
     c->stage_order = use.stage_order;
+    c->stage_shape = use.stage_shape;
     if (!r) {
         std::lock_guard<std::mutex> lk(g_mu);
         g_best[std::make_tuple(c->cfg.nx, c->cfg.ny, c->cfg.nz, c->cfg.precision)] = best;

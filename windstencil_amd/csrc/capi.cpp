@@ -453,12 +453,14 @@ int ws_ctx_create(const ws_config *cfg, ws_ctx **out)
     if (!c) return set_error(WS_ERR_NOMEM, "host allocation failed");
     c->cfg = *cfg;
     c->kp = make_kparams(*cfg);
-    int shape, sched, order, stage_order;
-    if (remembered_choice(cfg->nx, cfg->ny, cfg->nz, cfg->precision, &shape, &sched, &order, &stage_order)) {
+    int shape, sched, order, stage_order, stage_shape;
+    if (remembered_choice(cfg->nx, cfg->ny, cfg->nz, cfg->precision, &shape, &sched, &order, &stage_order,
+                          &stage_shape)) {
         c->tile_shape = shape;
         c->schedule = sched;
         c->dir_order = order;
         c->stage_order = stage_order;
+        c->stage_shape = stage_shape;
     }
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return hip_error(e, "hipGetDevice"); }
@@ -499,12 +501,14 @@ int ws_ctx_set_precision(ws_ctx *c, int32_t precision)
     if (!c) return set_error(WS_ERR_INVALID, "null context");
     if (precision != WS_PREC_FP64 && precision != WS_PREC_MIXED) return set_error(WS_ERR_INVALID, "unknown precision");
     c->cfg.precision = precision;
-    int shape, sched, order, stage_order;
-    if (remembered_choice(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision, &shape, &sched, &order, &stage_order)) {
+    int shape, sched, order, stage_order, stage_shape;
+    if (remembered_choice(c->cfg.nx, c->cfg.ny, c->cfg.nz, precision, &shape, &sched, &order, &stage_order,
+                          &stage_shape)) {
         c->tile_shape = shape;
         c->schedule = sched;
         c->dir_order = order;
         c->stage_order = stage_order;
+        c->stage_shape = stage_shape;
     }
     return WS_OK;
 }
@@ -567,6 +571,21 @@ int ws_ctx_get_stage_order(const ws_ctx *c, int32_t *order)
 {
     if (!c || !order) return set_error(WS_ERR_INVALID, "null argument");
     *order = c->stage_order;
+    return WS_OK;
+}
+
+int ws_ctx_set_stage_shape(ws_ctx *c, int32_t shape)
+{
+    if (!c) return set_error(WS_ERR_INVALID, "null context");
+    if (shape < -1 || shape >= WS_TILE_COUNT) return set_error(WS_ERR_INVALID, "unknown tile shape");
+    c->stage_shape = shape;
+    return WS_OK;
+}
+
+int ws_ctx_get_stage_shape(const ws_ctx *c, int32_t *shape)
+{
+    if (!c || !shape) return set_error(WS_ERR_INVALID, "null argument");
+    *shape = c->stage_shape;
     return WS_OK;
 }
 

@@ -21,7 +21,7 @@ long stage_launch_tiles(const Ctx &c)
     Box box;
     for (int d = 0; d < 3; ++d) box.s[d] = LAP;
     box.e[0] = c.kp.nx + LAP; box.e[1] = c.kp.ny + LAP; box.e[2] = c.kp.nz + LAP;
-    switch (c.tile_shape) {
+    switch (c.stage_shape >= 0 ? c.stage_shape : c.tile_shape) {
     case 0: return stage_tiles_n<0>(box, c.stage_order);
     case 1: return stage_tiles_n<1>(box, c.stage_order);
     case 2: return stage_tiles_n<2>(box, c.stage_order);

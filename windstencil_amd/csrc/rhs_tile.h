@@ -932,7 +932,7 @@ int launch_rhs_tile(const Ctx &c, ws_field d, ws_field u, ws_field v, ws_field w
     box.s[0] = (int)job.start.x; box.s[1] = (int)job.start.y; box.s[2] = (int)job.start.z;
     box.e[0] = (int)job.end.x; box.e[1] = (int)job.end.y; box.e[2] = (int)job.end.z;
     ws_metric_fields mf{};
-    const int shape = c.tile_shape;
+    const int shape = sa && c.stage_shape >= 0 ? c.stage_shape : c.tile_shape;
     const int order = sa ? c.stage_order : c.dir_order;
     KParams k = c.kp;
     k.off32 = prim_off32(k, d, u, v, w, cc) ? 1 : 0;

@@ -96,6 +96,8 @@ struct Ctx {
     // its best order can differ from the plain RHS's)
     int dir_order = WS_ORDER_XYZ;
     int stage_order = WS_ORDER_XYZ;
+    // tile shape of those stage-applying launches (-1: the context's tile_shape)
+    int stage_shape = -1;
     // RK3 steps apply each stage's update in the last RHS launch where they can (1), there
     // also derive the primitives of stages 1 and 2 (2), and within ws_rk3_run those of
     // stage 3 too, alternating the primitive sets from step to step (3)
@@ -205,10 +207,10 @@ int launch_stage_batch(const KParams &k, const AuxBlock *tab, int nb, long cell_
 int launch_cfl_batch(const AuxBlock *tab, int nb, const double *cfl_part, double cfl, double *dts, double *dt_dev,
                      hipStream_t s, const int2 *ranges = nullptr);
 
-// (tile shape, schedule, direction order, stage order) the autotuner found best for these
+// (tile shape, schedule, direction order, stage order and shape) the autotuner found best for these
 // extents and precision; false if none
 bool remembered_choice(int nx, int ny, int nz, int precision, int *shape, int *schedule, int *order,
-                       int *stage_order);
+                       int *stage_order, int *stage_shape);
 
 // error reporting (thread-local message)
 int set_error(int code, const std::string &msg);
