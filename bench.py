@@ -58,8 +58,8 @@ def parse():
                     help="rk3 and batched: force the stage order after tuning (0 x, y, z: z applies "
                          "the stage; 1 z, y, x: x applies it)")
     ap.add_argument("--stage-shape", type=int, default=-2,
-                    help="rk3: force the tile shape of the stage-applying launches after tuning "
-                         "(-1: the context's shape)")
+                    help="rk3 and batched: force the tile shape of the stage-applying launches after "
+                         "tuning (-1: the context's or batch's shape)")
     ap.add_argument("--workload", default="rhs", choices=["rhs", "batched", "rk3"],
                     help="rhs: the headline RHS line (default); batched: 48 uneven blocks; "
                          "rk3: SSP-RK3 steps of one box in a graph")
@@ -281,8 +281,11 @@ def bench_batched(args):
     W.check(L.ws_batch_get_schedule(b.handle, ctypes.byref(sched)))
     if args.order >= 0:
         W.check(L.ws_batch_set_stage_order(b.handle, args.order))
-    sorder = ctypes.c_int32()
+    if args.stage_shape >= -1:
+        W.check(L.ws_batch_set_stage_shape(b.handle, args.stage_shape))
+    sorder, sshape = ctypes.c_int32(), ctypes.c_int32()
     W.check(L.ws_batch_get_stage_order(b.handle, ctypes.byref(sorder)))
+    W.check(L.ws_batch_get_stage_shape(b.handle, ctypes.byref(sshape)))
     for _ in range(args.warmup):
         W.check(L.ws_batch_du_comput(b.handle, sh))
     torch.cuda.synchronize()
@@ -318,6 +321,7 @@ def bench_batched(args):
                   tile_shape=chosen.value, schedule=sched.value, autotune_ms=tune,
                   autotune_key="tile shape/schedule (0 grid of tiles, 1 persistent, 2 persistent + prefetch)",
                   rk3={"steps": nsteps, "graph": True, "seconds": round(rk3_s, 4), "stage_order": sorder.value,
+                       "stage_shape": sshape.value,
                        "mcell_updates_per_s": round(N * 3 * nsteps / rk3_s / 1e6, 2),
                        "mcell_steps_per_s": round(N * nsteps / rk3_s / 1e6, 2),
                        "note": "N x 3 stages x steps / wall time; CFL 0.5 from the per-block minimum each step"})

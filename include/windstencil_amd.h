@@ -452,6 +452,9 @@ int ws_batch_get_schedule(const ws_batch *batch, int32_t *schedule);
  * the z launch applies the stage, the default; WS_ORDER_ZYX: the x launch does). */
 int ws_batch_set_stage_order(ws_batch *batch, int32_t order);
 int ws_batch_get_stage_order(const ws_batch *batch, int32_t *order);
+/* As ws_ctx_set_stage_shape, for the batch (-1, the default: the batch's tile shape). */
+int ws_batch_set_stage_shape(ws_batch *batch, int32_t shape);
+int ws_batch_get_stage_shape(const ws_batch *batch, int32_t *shape);
 /* RHS of every block from ghost-filled d,u,v,w and cc (overwrites du). */
 int ws_batch_inviscid_rhs(ws_batch *batch, hipStream_t stream);
 /* ws_du_comput for every block: ghost fill, sound speed, then the batched RHS. */
@@ -464,7 +467,7 @@ int ws_batch_rk3_step(ws_batch *batch, const double *dt_dev, hipStream_t stream)
  * captures a whole step (CFL included) in one hipGraph and replays it. */
 int ws_batch_rk3_run(ws_batch *batch, int32_t nsteps, double cfl, int32_t use_graph, hipStream_t stream);
 /* Time every (tile shape, schedule) pair of the batched RHS and keep the fastest (as
- * ws_autotune); with the grid schedule and stage fusion on, then the stage order, timed on
+ * ws_autotune); with the grid schedule and stage fusion on, then the stage shape and order, timed on
  * a stage-2 RHS whose last launch writes into scratch the call allocates (a copy of every
  * block's f) and the blocks' second primitive sets (the blocks' f and primitives are left
  * as they were).  times_ms: WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major, or NULL. */

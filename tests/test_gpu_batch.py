@@ -214,6 +214,41 @@ def test_batch_alternating_primitive_sets_equal_level_2(ws, graph, cfl, nsteps, 
             assert np.array_equal(a[k], c[k]), (i, k)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_stage_shape_runs_the_stage_launches(ws, graph):
+    """A batch's stage-applying RHS runs in its stage shape: tile shape 1 with stage shape 4
+    gives, at stage fusion 3 (every RHS of a step applies a stage), the states of tile
+    shape 4 bit for bit (CFL step from the per-tile maxima of shape 4's x list), and the
+    plain RHS still runs in tile shape 1."""
+    extents = _extents(37, 4, 6, 30)
+    kinds = [k for k in KINDS if k != "rough"]
+    runs, rhs = [], []
+    for tile, sshape in ((W.WS_TILE_30x16_K2, W.WS_TILE_126x2_K2), (W.WS_TILE_126x2_K2, -1),
+                         (W.WS_TILE_30x16_K2, -1)):
+        b, _ = _batch(extents, 720, kinds=kinds, bc_planes=1, dt=2e-4)
+        L = W.lib()
+        W.check(L.ws_batch_set_schedule(b.handle, W.WS_SCHED_GRID))
+        W.check(L.ws_batch_set_tile_shape(b.handle, tile))
+        W.check(L.ws_batch_set_stage_shape(b.handle, sshape))
+        W.check(L.ws_batch_set_stage_order(b.handle, W.WS_ORDER_ZYX))
+        for e, st in zip(extents, b.states):
+            _to_cons(make_ctx(*e), st)
+        s = W.stream_handle()
+        W.check(L.ws_batch_du_comput(b.handle, s))
+        _sync()
+        rhs.append([_du(st) for st in b.states])
+        W.check(L.ws_batch_rk3_run(b.handle, 3, 0.5, graph, s))
+        b.sync_states()
+        _sync()
+        runs.append([{k: st.t[k].cpu().numpy() for k in ("f", "d", "u", "v", "w", "T", "cc", "P")}
+                     for st in b.states])
+        b.close()
+    for i in range(len(extents)):
+        for k in runs[0][i]:
+            assert np.array_equal(runs[0][i][k], runs[1][i][k]), (i, k)
+        assert np.array_equal(rhs[0][i], rhs[2][i]), i   # the plain RHS: tile shape 1
+
+
 @pytest.mark.parametrize("fusion", [0, 2, 3])
 @pytest.mark.parametrize("graph", [0, 1])
 def test_batch_run_cfl_from_stage_maxima_bitwise(ws, graph, fusion):

@@ -163,6 +163,8 @@ _SIGS = {
     "ws_batch_get_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_batch_set_stage_order": (ctypes.c_int, [ctypes.c_void_p, _i32]),
     "ws_batch_get_stage_order": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
+    "ws_batch_set_stage_shape": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "ws_batch_get_stage_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_i32)]),
     "ws_batch_inviscid_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ws_batch_du_comput": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ws_batch_cfl_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),

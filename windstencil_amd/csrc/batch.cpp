@@ -32,6 +32,8 @@ struct ws_batch {
     // which launch of a stage-applying RHS applies the stage (WS_ORDER_*: the last
     // direction; ws_batch_autotune picks)
     int stage_order = WS_ORDER_XYZ;
+    // and its tile shape (-1: tile_shape)
+    int stage_shape = -1;
     // KParams::off32 for the RHS inputs of d_blocks (every block's own fields) and of
     // d_blocks2 (the second primitive sets): every block's five fields pass prim_off32
     int off32_own = 0, off32_second = 0;
@@ -89,17 +91,21 @@ void block_tiles(int b, const int n[3], int shape, int dir, std::vector<int4> &o
             for (int x = 0; x < gx; ++x) out.push_back(make_int4(b, x, y, z));
 }
 
-BatchTiles tiles_of(const ws_batch *b)
+// the tile shape of the batch's stage-applying RHS
+int stage_shape_of(const ws_batch *b) { return b->stage_shape >= 0 ? b->stage_shape : b->tile_shape; }
+
+// the launch tables of the batch's RHS, or with `stage` of its stage-applying RHS
+BatchTiles tiles_of(const ws_batch *b, bool stage = false)
 {
     BatchTiles t;
-    t.shape = b->tile_shape;
+    t.shape = stage ? stage_shape_of(b) : b->tile_shape;
     t.schedule = b->schedule;
     t.blocks = b->d_blocks;
     t.stage = nullptr;
     t.stage_order = b->stage_order;
     for (int d = 0; d < 3; ++d) {
-        t.tiles[d] = b->d_tiles[b->tile_shape][d];
-        t.ntiles[d] = b->ntiles[b->tile_shape][d];
+        t.tiles[d] = b->d_tiles[t.shape][d];
+        t.ntiles[d] = b->ntiles[t.shape][d];
     }
     return t;
 }
@@ -116,7 +122,7 @@ int batch_rhs(ws_batch *b, hipStream_t s, const BatchStage *stage = nullptr, con
     }
     KParams k = b->ctx[0]->kp;   // physics constants and metrics are the batch's
     k.off32 = table && table == b->d_blocks2 ? b->off32_second : b->off32_own;
-    BatchTiles t = tiles_of(b);
+    BatchTiles t = tiles_of(b, stage != nullptr);
     t.stage = stage;
     if (table) t.blocks = table;
     return b->precision == WS_PREC_MIXED ? launch_rhs_batch_mixed(k, t, s) : launch_rhs_batch_tiled(k, t, s);
@@ -386,6 +392,21 @@ int ws_batch_set_stage_order(ws_batch *b, int32_t order)
     return WS_OK;
 }
 
+int ws_batch_set_stage_shape(ws_batch *b, int32_t shape)
+{
+    if (!b) return set_error(WS_ERR_INVALID, "null batch");
+    if (shape < -1 || shape >= WS_TILE_COUNT) return set_error(WS_ERR_INVALID, "unknown tile shape");
+    b->stage_shape = shape;
+    return WS_OK;
+}
+
+int ws_batch_get_stage_shape(const ws_batch *b, int32_t *shape)
+{
+    if (!b || !shape) return set_error(WS_ERR_INVALID, "null argument");
+    *shape = b->stage_shape;
+    return WS_OK;
+}
+
 int ws_batch_get_stage_order(const ws_batch *b, int32_t *order)
 {
     if (!b || !order) return set_error(WS_ERR_INVALID, "null argument");
@@ -606,7 +627,7 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
         // the previous step left its maxima per tile (alternating) or in the batch's buffer
         if (alt)
             return launch_cfl_batch(b->d_aux, nb, b->d_cfl_tiles, cfl, b->d_dts, dt_dev, ss,
-                                    b->d_tile_ranges[b->tile_shape][b->stage_order]);
+                                    b->d_tile_ranges[stage_shape_of(b)][b->stage_order]);
         if (b->cfl_in_batch) return launch_cfl_batch(b->d_aux, nb, b->d_cfl_part, cfl, b->d_dts, dt_dev, ss);
         for (size_t i = 0; i < b->ctx.size(); ++i) {
             ws_ctx *c = b->ctx[i];
@@ -667,13 +688,15 @@ int ws_batch_rk3_run(ws_batch *b, int32_t nsteps, double cfl, int32_t use_graph,
     return r;
 }
 
-// The batch's stage order, timed as ws_autotune times a context's: the batched RHS of an
+// The batch's stage shape and order, timed as ws_autotune times a context's: the batched RHS of an
 // RK3 stage 2 whose last launch writes every block's new f into scratch allocated here (a
 // copy of the block's f, which is the stage's fn and is only read) and, where the blocks
 // keep second primitive sets, the primitives, sound speed and their halo into those (the
-// sets a run's first stage overwrites anyway).  The faster order, or -1 when the scratch
-// cannot be had (the order then stays as it was).
-static int batch_tune_stage_order(ws_batch *b, int reps, hipEvent_t e0, hipEvent_t e1, hipStream_t s, int &r)
+// sets a run's first stage overwrites anyway), in every tile shape and both orders.  Sets
+// *shape and *order to the fastest; false when the scratch cannot be had (both then stay
+// as they were) or on an error (r set).
+static bool batch_tune_stage(ws_batch *b, int reps, hipEvent_t e0, hipEvent_t e1, hipStream_t s, int &r,
+                             int *shape_out, int *order_out)
 {
     const bool prims = b->ctx[0]->fuse_stage > 1 && prim_tables(b, s);
     const size_t nb = b->st.size();
@@ -711,28 +734,34 @@ static int batch_tune_stage_order(ws_batch *b, int reps, hipEvent_t e0, hipEvent
     bs.prims = prims ? 1 : 0;
     bs.planes = b->base.bc_planes != 0;
     bs.halo = bs.prims;
-    const int saved = b->stage_order;
-    int best = -1;
+    const int saved = b->stage_order, saved_shape = b->stage_shape;
     float best_ms = 1e30f;
-    for (int o = 0; o < WS_ORDER_COUNT && !r; ++o) {
-        b->stage_order = o;
-        for (int i = 0; i < 2 && !r; ++i) r = batch_rhs(b, s, &bs, d_tab);
-        if (r) break;
-        hipEventRecord(e0, s);
-        for (int i = 0; i < reps && !r; ++i) r = batch_rhs(b, s, &bs, d_tab);
-        hipEventRecord(e1, s);
-        if (r) break;
-        hipError_t e = hipEventSynchronize(e1);
-        if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) { best_ms = ms; best = o; }
-    }
+    for (int sh = 0; sh < WS_TILE_COUNT && !r; ++sh)
+        for (int o = 0; o < WS_ORDER_COUNT && !r; ++o) {
+            b->stage_shape = sh;
+            b->stage_order = o;
+            for (int i = 0; i < 2 && !r; ++i) r = batch_rhs(b, s, &bs, d_tab);
+            if (r) break;
+            hipEventRecord(e0, s);
+            for (int i = 0; i < reps && !r; ++i) r = batch_rhs(b, s, &bs, d_tab);
+            hipEventRecord(e1, s);
+            if (r) break;
+            hipError_t e = hipEventSynchronize(e1);
+            if (e != hipSuccess) { r = hip_error(e, "autotune sync"); break; }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                *shape_out = sh;
+                *order_out = o;
+            }
+        }
     b->stage_order = saved;
+    b->stage_shape = saved_shape;
     hipStreamSynchronize(s);
     hipFree(d_tab);
     hipFree(scratch);
-    return r ? -1 : best;
+    return !r;
 }
 
 int ws_batch_autotune(ws_batch *b, int32_t reps, float *times_ms, hipStream_t s)
@@ -768,8 +797,11 @@ int ws_batch_autotune(ws_batch *b, int32_t reps, float *times_ms, hipStream_t s)
     if (!r && best_sched == WS_SCHED_GRID && !b->base.viscous && b->ctx[0]->fuse_stage) {
         b->tile_shape = best_shape;
         b->schedule = best_sched;
-        const int o = batch_tune_stage_order(b, reps, e0, e1, s, r);
-        if (o >= 0) b->stage_order = o;
+        int sh = b->stage_shape, o = b->stage_order;
+        if (batch_tune_stage(b, reps, e0, e1, s, r, &sh, &o)) {
+            b->stage_shape = sh;
+            b->stage_order = o;
+        }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
