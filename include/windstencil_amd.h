@@ -292,7 +292,7 @@ int ws_cfl_dt(ws_ctx *ctx, const ws_state *st, double cfl, double *dt_dev, hipSt
  * `stream`), keep the fastest for the context and remember it for contexts of the same
  * extents and precision.  When RK3 steps of that choice apply their stages in the last RHS
  * launch (ws_ctx_set_stage_fusion), the stage order and tile shape are then timed the same
- * way (every shape in both orders): the RHS of an RK3 stage 2 whose last launch writes the
+ * way (every shape in both orders, 3 * reps launches each): the RHS of an RK3 stage 2 whose last launch writes the
  * new f, primitives, sound speed and halo into scratch the call allocates and frees (the
  * state's f is read, not written).  times_ms
  * (WS_TILE_COUNT * WS_SCHED_COUNT entries, shape-major, each the faster of the two orders)

@@ -183,8 +183,9 @@ extern "C" int ws_autotune(ws_ctx *c, const ws_state *st, int32_t reps, float *t
         c->tile_shape = best.shape;
         c->schedule = best.schedule;
         c->dir_order = best.order;
+        // the stage candidates differ by 1-3 %: three times the repetitions
         int sh = best.stage_shape, o = best.stage_order;
-        if (fused_stage_ok(c) && tune_stage(c, st, job, reps, e0, e1, s, r, &sh, &o)) {
+        if (fused_stage_ok(c) && tune_stage(c, st, job, 3 * reps, e0, e1, s, r, &sh, &o)) {
             best.stage_shape = sh;
             best.stage_order = o;
         }

@@ -797,8 +797,9 @@ int ws_batch_autotune(ws_batch *b, int32_t reps, float *times_ms, hipStream_t s)
     if (!r && best_sched == WS_SCHED_GRID && !b->base.viscous && b->ctx[0]->fuse_stage) {
         b->tile_shape = best_shape;
         b->schedule = best_sched;
+        // the stage candidates differ by 1-3 %: three times the repetitions
         int sh = b->stage_shape, o = b->stage_order;
-        if (batch_tune_stage(b, reps, e0, e1, s, r, &sh, &o)) {
+        if (batch_tune_stage(b, 3 * reps, e0, e1, s, r, &sh, &o)) {
             b->stage_shape = sh;
             b->stage_order = o;
         }
